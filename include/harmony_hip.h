@@ -1,0 +1,164 @@
+/*
+ * libharmony_hip -- C ABI of the MI355X (gfx950) Harmony integration path.
+ *
+ * Every entry point replaces a native call that rapids_singlecell's Python
+ * driver makes through nanobind; the replaced interface is cited above each
+ * declaration (paths relative to src/rapids_singlecell/ in the reference).
+ *
+ * Conventions
+ *   - Plain pointers and sizes only.  Array pointers are device pointers
+ *     (hipMalloc / torch-ROCm memory), C-contiguous, row-major.  Pointers typed
+ *     `void*` hold float or double data as selected by `dtype`.
+ *   - The caller owns all memory, including the workspace whose size the
+ *     *_workspace_bytes queries return.  Nothing is freed by the library.
+ *   - `stream` is a hipStream_t (NULL = default stream).  Work is enqueued on
+ *     it; entry points that must report a host value synchronise it.
+ *   - Return value: HZ_OK or a negative HZ_ERR_* code; hz_last_error()
+ *     describes the last failure of the calling thread.
+ *   - Calls that share a workspace must not overlap.
+ */
+#ifndef HARMONY_HIP_H
+#define HARMONY_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HZ_OK 0
+#define HZ_ERR_ARG (-1)
+#define HZ_ERR_HIP (-2)
+#define HZ_ERR_LAUNCH (-3)
+#define HZ_ERR_UNSUPPORTED (-4)
+#define HZ_ERR_WORKSPACE (-5)
+
+/* element size selects the arithmetic type */
+#define HZ_F32 4
+#define HZ_F64 8
+
+#define HZ_PREC_NATIVE 0     /* GEMMs in the storage type */
+#define HZ_PREC_SPLIT_BF16 1 /* FP32 storage, split-BF16 products (reserved) */
+
+#define HZ_CORR_BATCHED 0
+#define HZ_CORR_FAST 1
+#define HZ_CORR_ORIGINAL 2
+
+/* Problem shape shared by the job-level entry points. */
+typedef struct hz_dims {
+  int n_cells;
+  int n_pcs;
+  int n_clusters;
+  int n_batches;
+  int block_size; /* int(n_cells * block_proportion), >= 1 */
+  int dtype;      /* HZ_F32 or HZ_F64 */
+  int precision;  /* HZ_PREC_* */
+} hz_dims;
+
+int hz_version(void);
+const char* hz_last_error(void);
+
+/* Workspace for hz_init_clusters / hz_clustering_loop / hz_correction.
+ * Replaces _allocate_clustering_workspace (_harmony/__init__.py:356-384),
+ * get_cub_sort_temp_bytes (_cuda/harmony/clustering/clustering.cu:48-59) and
+ * the per-call workspace of _correction_batched (_harmony/__init__.py:648-656).
+ * Returns 0 for an unsupported shape (see hz_last_error). */
+size_t hz_workspace_bytes(const hz_dims* dims);
+
+/* L2 row normalisation, dst = src * min(rsqrt(sum src^2), 1e12).
+ * Replaces _harmony_normalize_cuda.l2_row_normalize
+ * (_cuda/harmony/normalize/normalize.cu:35-68, kernels_normalize.cuh:6-54). */
+int hz_l2_normalize(int dtype, const void* src, void* dst, long long n_rows, int n_cols,
+                    void* stream);
+
+/* Key-sorted cell order of one clustering iteration: order[r] = cell of rank r
+ * for keys pcg(i ^ seed).  Replaces pcg_hash_kernel + iota_kernel + CUB
+ * SortPairs (_cuda/harmony/clustering/clustering.cu:292-305,
+ * kernels_clustering.cuh:106-126). */
+size_t hz_block_order_workspace_bytes(int n_cells);
+int hz_block_order(int n_cells, uint32_t seed, int32_t* order, void* ws, size_t ws_bytes,
+                   void* stream);
+
+/* Initial R, E, O and objective from centroids Y0 (n_clusters x n_pcs).
+ * Replaces the body of _initialize_centroids after k-means
+ * (_harmony/__init__.py:323-351): _calc_R (_fuses.py:6-8), L1 normalise,
+ * _outer_cp, _scatter_add_cp and compute_objective.  obj_out: device scalar. */
+int hz_init_clusters(const hz_dims* dims, const void* Z_norm, const void* Y0,
+                     const int32_t* cats, const int32_t* cat_offsets,
+                     const int32_t* cell_indices, const void* Pr_b, const void* theta,
+                     double sigma, int stabilized, void* R, void* E, void* O, void* obj_out,
+                     void* ws, size_t ws_bytes, void* stream);
+
+/* One Harmony iteration's clustering step; R, E, O are updated in place and
+ * the final objective is written to the device scalar last_obj; *n_iter
+ * receives the number of clustering iterations executed.
+ * Replaces _harmony_clustering_cuda.clustering_loop
+ * (_cuda/harmony/clustering/clustering.cu:241-393, binding :397-469). */
+int hz_clustering_loop(const hz_dims* dims, const void* Z_norm, void* R, void* E, void* O,
+                       const void* Pr_b, const int32_t* cats, const void* theta, double sigma,
+                       double tol, int max_iter, uint32_t seed, int stabilized, void* last_obj,
+                       int* n_iter, void* ws, size_t ws_bytes, void* stream);
+
+/* Ridge correction from the ORIGINAL X: lambda_kb (written to lambda_kb_out,
+ * n_batches x n_clusters), Z_hat and its L2-normalised rows Z_norm.
+ * Replaces _compute_lambda_kb (_harmony/__init__.py:454-480) and
+ * _harmony_correction_batched_cuda.correction_batched
+ * (_cuda/harmony/correction/correction_batched.cu:18-130, binding :135-164),
+ * _harmony_correction_cuda.correction_fast (correction_fast.cu:23-109) and
+ * _correction_original (_harmony/__init__.py:532-570), plus the following
+ * _normalize_cp(Z_hat) (_harmony/__init__.py:276). */
+int hz_correction(const hz_dims* dims, const void* X, const void* R, const void* O,
+                  const void* E, const void* N_b, const int32_t* cat_offsets,
+                  const int32_t* cell_indices, int method, int dynamic_lambda, double alpha,
+                  int has_threshold, double threshold, double ridge_lambda,
+                  void* lambda_kb_out, void* Z_hat, void* Z_norm, void* ws, size_t ws_bytes,
+                  void* stream);
+
+/* Three-term objective of given R and similarities; result copied to *obj_host.
+ * Replaces _harmony_clustering_cuda.compute_objective
+ * (_cuda/harmony/clustering/clustering.cu:188-237, binding :472-489). */
+size_t hz_objective_workspace_bytes(void);
+int hz_objective(int dtype, const void* R, const void* sim, const void* O, const void* E,
+                 const void* theta, double sigma, int n_cells, int n_clusters, int n_batches,
+                 int stabilized, double* obj_host, void* ws, size_t ws_bytes, void* stream);
+
+/* Penalty ((E+1)/(O+E+1))^theta (stabilized) or ((E+1)/(O+1))^theta.
+ * Replaces _harmony_pen_cuda.penalty (_cuda/harmony/pen/pen.cu:13-25,
+ * kernels_pen.cuh:8-20). */
+int hz_penalty(int dtype, const void* E, const void* O, const void* theta, void* pen,
+               int n_batches, int n_clusters, int stabilized, void* stream);
+
+/* R_out[r] = L1(exp(term (1 - sim[idx[r]])) * pen[cats[r]]).
+ * Replaces the test binding _harmony_pen_cuda.fused_pen_norm_int
+ * (_cuda/harmony/pen/pen.cu:102-106, kernels_pen.cuh:27-82). */
+int hz_fused_pen_norm(int dtype, const void* sim, const void* pen, const int32_t* cats,
+                      const int32_t* idx, void* R_out, double term, int n_rows, int n_cols,
+                      void* stream);
+
+/* lambda_kb (n_batches x n_clusters). Replaces _compute_lambda_kb
+ * (_harmony/__init__.py:454-480). */
+int hz_lambda_kb(int dtype, const void* E, const void* O, const void* N_b, int n_batches,
+                 int n_clusters, int dynamic_lambda, double alpha, int has_threshold,
+                 double threshold, double ridge_lambda, void* out, void* stream);
+
+/* Closed-form inverses, n_clusters x (n_batches+1)^2.  Replaces the test
+ * binding _harmony_correction_cuda.compute_inv_mat
+ * (_cuda/harmony/correction/correction_fast.cu:142-159,
+ * kernels_correction_fast.cuh:24-123). */
+int hz_inv_mats(int dtype, const void* O, const void* lambda_kb, void* inv, int n_batches,
+                int n_clusters, void* stream);
+
+/* Seeded Lloyd k-means on Z_norm producing the initial centroids Y_out
+ * (k x d).  Replaces cuML KMeans(init="k-means||", n_init=1, max_iter=25)
+ * (_harmony/__init__.py:315-323) with this library's own initialiser: seeds
+ * are the first k cells of the pcg order for `seed`. */
+size_t hz_kmeans_workspace_bytes(int dtype, long long n_cells, int n_pcs, int k);
+int hz_kmeans(int dtype, const void* Z_norm, long long n_cells, int n_pcs, int k, uint32_t seed,
+              int max_iter, void* Y_out, void* ws, size_t ws_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HARMONY_HIP_H */

@@ -1,0 +1,987 @@
+// libharmony_hip: extern "C" entry points and host orchestration.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "hz_correct.hpp"
+
+namespace hz {
+
+// ---------------------------------------------------------------- errors
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+// ---------------------------------------------------------------- device info
+
+static int g_lds[64];
+static int g_cus[64];
+
+static int current_device() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+  return std::max(0, std::min(dev, 63));
+}
+
+int device_lds_limit() {
+  int dev = current_device();
+  if (!g_lds[dev]) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+        v <= 0)
+      v = 65536;
+    g_lds[dev] = v;
+  }
+  return g_lds[dev];
+}
+
+int device_cu_count() {
+  int dev = current_device();
+  if (!g_cus[dev]) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        v <= 0)
+      v = 256;
+    g_cus[dev] = v;
+  }
+  return g_cus[dev];
+}
+
+static int pick_ct(int K) {
+  if (K <= 64) return 4;
+  if (K <= 128) return 8;
+  if (K <= 256) return 16;
+  return 0;
+}
+
+static int pick_dt(int d) {
+  if (d <= 32) return 2;
+  if (d <= 64) return 4;
+  if (d <= 128) return 8;
+  return 0;
+}
+
+static int fixed_shift(unsigned long long max_abs) {
+  return std::min(52, 61 - bit_length(max_abs));
+}
+
+// ---------------------------------------------------------------- layout
+
+struct Layout {
+  int n, d, K, B, bs, esz;
+  int CT, Kp, DT, Dp;
+  int chunk, chunk_p, G, cpw;
+  PermDims pd;
+  PermDims sd;  // static row chunks (cells grouped by batch)
+  PermDims spd; // static phi chunks
+  // clustering
+  PermBufs pb;
+  ChunkTab ct;
+  int* nch_tmp;
+  ChunkTab st;
+  int* nch_tmp_s;
+  ChunkTab sp;
+  int* nch_tmp_p;
+  long long* acc_in;
+  long long* acc_out;
+  Ctrl* ctrl;
+  Ctrl* ctrl_static;  // never set done: used by the correction pass
+  int* counters;      // nblk + 4
+  void* Ypart;
+  void* Y;
+  void* YT;
+  void* pen;
+  double* scratch;
+  double* cs;
+  // correction
+  void* phipart;
+  double* Phi;
+  long long* rsum;
+  void* lam;
+  void* Wt;
+  size_t bytes;
+  double scale_o, scale_obj;
+};
+
+static int make_layout(const hz_dims* dm, void* base, Layout& L) {
+  HZ_RET_IF(!dm, HZ_ERR_ARG, "dims is NULL");
+  HZ_RET_IF(dm->dtype != HZ_F32 && dm->dtype != HZ_F64, HZ_ERR_ARG, "dtype must be 4 or 8");
+  HZ_RET_IF(dm->n_cells < 1 || dm->n_pcs < 1 || dm->n_clusters < 1 || dm->n_batches < 1,
+            HZ_ERR_ARG, "empty problem (n_cells=%d n_pcs=%d n_clusters=%d n_batches=%d)",
+            dm->n_cells, dm->n_pcs, dm->n_clusters, dm->n_batches);
+  HZ_RET_IF(dm->block_size < 1, HZ_ERR_ARG,
+            "block_size must be >= 1 (int(n_cells * block_proportion) = %d)", dm->block_size);
+  HZ_RET_IF(dm->precision != HZ_PREC_NATIVE, HZ_ERR_UNSUPPORTED,
+            "precision %d is not available in this build", dm->precision);
+  L.n = dm->n_cells;
+  L.d = dm->n_pcs;
+  L.K = dm->n_clusters;
+  L.B = dm->n_batches;
+  L.bs = std::min(dm->block_size, dm->n_cells);
+  L.esz = dm->dtype;
+  L.CT = pick_ct(L.K);
+  L.DT = pick_dt(L.d);
+  HZ_RET_IF(!L.CT, HZ_ERR_UNSUPPORTED, "n_clusters=%d > 256 is not supported", L.K);
+  HZ_RET_IF(!L.DT, HZ_ERR_UNSUPPORTED, "n_pcs=%d > 128 is not supported", L.d);
+  HZ_RET_IF(L.B > 16384, HZ_ERR_UNSUPPORTED, "n_batches=%d > 16384 is not supported", L.B);
+  L.Kp = 16 * L.CT;
+  L.Dp = 16 * L.DT;
+  L.chunk = (L.bs <= 32768) ? 64 : 128;
+  {
+    long long target = ceil_div(L.n, 512);
+    int cp = 64;
+    while (cp < target && cp < 4096) cp <<= 1;
+    L.chunk_p = cp;
+  }
+  L.pd = perm_dims(L.n, L.bs, L.B, L.chunk);
+  L.sd = perm_dims(L.n, L.n, L.B, L.chunk);
+  L.spd = perm_dims(L.n, L.n, L.B, L.chunk_p);
+  L.G = (int)std::min<long long>(512, L.pd.maxch);
+  L.cpw = (int)ceil_div(L.pd.maxch, L.G);
+  L.G = (int)ceil_div(L.pd.maxch, L.cpw);
+  const long long nseg = (long long)L.pd.nblk * L.B;
+  const long long BK = (long long)L.B * L.K;
+  const long long KD = (long long)L.K * L.d;
+
+  Carve c(base);
+  carve_perm(c, L.pd, L.pb, L.ct);
+  L.nch_tmp = c.take<int>(nseg + 1);
+  {
+    PermBufs dummy;
+    (void)dummy;
+    L.st.order = c.take<int>(L.n);
+    L.st.seg_start = c.take<int>(L.B + 1);
+    L.st.seg_ch = c.take<int>(L.B + 1);
+    L.st.ch_start = c.take<int>(L.sd.maxch);
+    L.st.ch_len = c.take<int>(L.sd.maxch);
+    L.st.ch_seg = c.take<int>(L.sd.maxch);
+    L.nch_tmp_s = c.take<int>(L.B + 1);
+    L.sp.order = c.take<int>(L.n);
+    L.sp.seg_start = c.take<int>(L.B + 1);
+    L.sp.seg_ch = c.take<int>(L.B + 1);
+    L.sp.ch_start = c.take<int>(L.spd.maxch);
+    L.sp.ch_len = c.take<int>(L.spd.maxch);
+    L.sp.ch_seg = c.take<int>(L.spd.maxch);
+    L.nch_tmp_p = c.take<int>(L.B + 1);
+  }
+  L.acc_in = c.take<long long>(nseg * L.K);
+  L.acc_out = c.take<long long>(nseg * L.K);
+  L.ctrl = c.take<Ctrl>(1);
+  L.ctrl_static = c.take<Ctrl>(1);
+  L.counters = c.take<int>(L.pd.nblk + 4);
+  L.Ypart = c.take<char>((size_t)L.G * KD * L.esz);
+  L.Y = c.take<char>((size_t)KD * L.esz);
+  L.YT = c.take<char>((size_t)L.d * L.Kp * L.esz);
+  L.pen = c.take<char>((size_t)BK * L.esz);
+  L.scratch = c.take<double>(BK);
+  L.cs = c.take<double>(L.K);
+  L.phipart = c.take<char>((size_t)L.spd.maxch * KD * L.esz);
+  L.Phi = c.take<double>(BK * L.d);
+  L.rsum = c.take<long long>(BK);
+  L.lam = c.take<char>((size_t)BK * L.esz);
+  L.Wt = c.take<char>((size_t)BK * L.d * L.esz);
+  L.bytes = align_up(c.off, 256);
+  L.scale_o = std::ldexp(1.0, fixed_shift((unsigned long long)L.n + 1));
+  L.scale_obj = std::ldexp(1.0, fixed_shift(16ull * (unsigned long long)L.n + 16));
+  return HZ_OK;
+}
+
+template <typename T>
+static EpiArgs<T> make_epi(const Layout& L, T* O, T* E, const T* Pr, const T* theta, double sigma,
+                           double tol, int stabilized, T* last_obj) {
+  EpiArgs<T> e;
+  e.O = O;
+  e.E = E;
+  e.Pr = Pr;
+  e.theta = theta;
+  e.pen = reinterpret_cast<T*>(L.pen);
+  e.acc_in = L.acc_in;
+  e.acc_out = L.acc_out;
+  e.scratch = L.scratch;
+  e.cs = L.cs;
+  e.ctrl = L.ctrl;
+  e.last_obj = last_obj;
+  e.B = L.B;
+  e.K = L.K;
+  e.nblk = L.pd.nblk;
+  e.sigma = T(sigma);
+  e.tol = T(tol);
+  e.stabilized = stabilized;
+  e.inv_scale_o = 1.0 / L.scale_o;
+  e.inv_scale_obj = 1.0 / L.scale_obj;
+  return e;
+}
+
+static int check_ws(const Layout& L, void* ws, size_t ws_bytes) {
+  HZ_RET_IF(!ws, HZ_ERR_WORKSPACE, "workspace is NULL");
+  HZ_RET_IF(ws_bytes < L.bytes, HZ_ERR_WORKSPACE, "workspace too small: %zu < %zu bytes",
+            ws_bytes, L.bytes);
+  HZ_RET_IF(reinterpret_cast<uintptr_t>(ws) % 256 != 0, HZ_ERR_WORKSPACE,
+            "workspace must be 256-byte aligned");
+  return HZ_OK;
+}
+
+template <typename T>
+static size_t row_lds_bytes(const Layout& L) {
+  return (size_t)L.d * L.Kp * sizeof(T) + (size_t)L.d * kZtStride * sizeof(T) +
+         (size_t)(kBlock / kWave) * L.Kp * sizeof(double);
+}
+
+template <typename T, int CT>
+static int launch_row_ct(const RowArgs<T>& a, int grid, size_t lds, hipStream_t s) {
+  auto kern = row_kernel<T, CT>;
+  if (lds > 65536)
+    HZ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), lds, s, a);
+  HZ_LAUNCHED("row_kernel");
+  return HZ_OK;
+}
+
+template <typename T>
+static int launch_row(const Layout& L, const RowArgs<T>& a, int grid, hipStream_t s) {
+  const size_t lds = row_lds_bytes<T>(L);
+  HZ_RET_IF(lds > (size_t)device_lds_limit(), HZ_ERR_UNSUPPORTED,
+            "row kernel needs %zu bytes of LDS (limit %d): reduce n_pcs or n_clusters", lds,
+            device_lds_limit());
+  switch (L.CT) {
+    case 4: return launch_row_ct<T, 4>(a, grid, lds, s);
+    case 8: return launch_row_ct<T, 8>(a, grid, lds, s);
+    default: return launch_row_ct<T, 16>(a, grid, lds, s);
+  }
+}
+
+template <typename T, int KT>
+static int launch_centroid_kt(const Layout& L, const CentArgs<T>& a, int grid, hipStream_t s) {
+  switch (L.DT) {
+    case 2: hipLaunchKernelGGL((centroid_kernel<T, KT, 2>), dim3(grid), dim3(kBlock), 0, s, a); break;
+    case 4: hipLaunchKernelGGL((centroid_kernel<T, KT, 4>), dim3(grid), dim3(kBlock), 0, s, a); break;
+    default: hipLaunchKernelGGL((centroid_kernel<T, KT, 8>), dim3(grid), dim3(kBlock), 0, s, a); break;
+  }
+  HZ_LAUNCHED("centroid_kernel");
+  return HZ_OK;
+}
+
+template <typename T>
+static int launch_centroid(const Layout& L, const CentArgs<T>& a, int grid, hipStream_t s) {
+  const size_t lds = (size_t)kTileRows * (L.Kp + L.Dp) * sizeof(T);
+  HZ_RET_IF(lds > (size_t)device_lds_limit(), HZ_ERR_UNSUPPORTED,
+            "centroid kernel needs %zu bytes of LDS", lds);
+  switch (L.CT) {
+    case 4: return launch_centroid_kt<T, 4>(L, a, grid, s);
+    case 8: return launch_centroid_kt<T, 8>(L, a, grid, s);
+    default: return launch_centroid_kt<T, 16>(L, a, grid, s);
+  }
+}
+
+static uint32_t* g_pinned_flag = nullptr;
+
+static int read_done(const Ctrl* ctrl, hipStream_t s, int* done) {
+  if (!g_pinned_flag) HZ_HIP(hipHostMalloc(reinterpret_cast<void**>(&g_pinned_flag), 64));
+  HZ_HIP(hipMemcpyAsync(g_pinned_flag, &ctrl->done, sizeof(int), hipMemcpyDeviceToHost, s));
+  HZ_HIP(hipStreamSynchronize(s));
+  *done = (int)g_pinned_flag[0];
+  return HZ_OK;
+}
+
+// ---------------------------------------------------------------- jobs
+
+template <typename T>
+static int init_impl(const hz_dims* dm, const T* Z, const T* Y0, const int* cats,
+                     const int* cat_offsets, const int* cell_indices, const T* Pr, const T* theta,
+                     double sigma, int stabilized, T* R, T* E, T* O, T* obj_out, void* ws,
+                     size_t ws_bytes, hipStream_t s) {
+  Layout L;
+  HZ_TRY(make_layout(dm, nullptr, L));
+  HZ_TRY(check_ws(L, ws, ws_bytes));
+  HZ_TRY(make_layout(dm, ws, L));
+  (void)cats;
+  HZ_TRY(launch_static_chunks(L.n, L.B, L.chunk, cat_offsets, cell_indices, L.st, L.nch_tmp_s, s));
+  hipLaunchKernelGGL(ctrl_reset_kernel, dim3(1), dim3(64), 0, s, L.ctrl);
+  HZ_LAUNCHED("ctrl_reset");
+  HZ_TRY(fill<long long>(L.acc_out, (long long)L.B * L.K, 0, s));
+  HZ_TRY(fill<int>(L.counters, L.pd.nblk + 4, 0, s));
+  hipLaunchKernelGGL(centroids_to_yt_kernel<T>, dim3(1), dim3(kBlock), 0, s, Y0,
+                     reinterpret_cast<T*>(L.YT), L.K, L.d, L.Kp);
+  HZ_LAUNCHED("centroids_to_yt");
+  RowArgs<T> a;
+  a.Z = Z;
+  a.YT = reinterpret_cast<const T*>(L.YT);
+  a.R = R;
+  a.pen = nullptr;
+  a.order = L.st.order;
+  a.seg_ch = L.st.seg_ch;
+  a.ch_start = L.st.ch_start;
+  a.ch_len = L.st.ch_len;
+  a.ch_seg = L.st.ch_seg;
+  a.acc_out = L.acc_out;
+  a.counter = L.counters + L.pd.nblk + 1;
+  a.seg_base = 0;
+  a.block = -1;
+  a.term = T(-2.0 / sigma);
+  a.n = L.n;
+  a.d = L.d;
+  a.K = L.K;
+  a.Kp = L.Kp;
+  a.vec_store = (L.K % 4 == 0) ? 1 : 0;
+  a.scale_o = L.scale_o;
+  a.scale_obj = L.scale_obj;
+  a.obj_out = obj_out;
+  a.epi = make_epi<T>(L, O, E, Pr, theta, sigma, 0.0, stabilized, nullptr);
+  const int grid = (int)L.sd.maxch;
+  return launch_row<T>(L, a, grid, s);
+}
+
+template <typename T>
+static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, const T* Pr,
+                           const int* cats, const T* theta, double sigma, double tol, int max_iter,
+                           uint32_t seed, int stabilized, T* last_obj, int* n_iter, void* ws,
+                           size_t ws_bytes, hipStream_t s) {
+  Layout L;
+  HZ_TRY(make_layout(dm, nullptr, L));
+  HZ_TRY(check_ws(L, ws, ws_bytes));
+  HZ_TRY(make_layout(dm, ws, L));
+  if (n_iter) *n_iter = 0;
+  if (max_iter <= 0) {
+    if (last_obj) HZ_TRY(fill<T>(last_obj, 1, T(0), s));
+    return HZ_OK;
+  }
+  const long long nseg = (long long)L.pd.nblk * L.B;
+  hipLaunchKernelGGL(ctrl_reset_kernel, dim3(1), dim3(64), 0, s, L.ctrl);
+  HZ_LAUNCHED("ctrl_reset");
+  HZ_TRY(fill<long long>(L.acc_in, nseg * L.K, 0, s));
+  HZ_TRY(fill<long long>(L.acc_out, nseg * L.K, 0, s));
+  HZ_TRY(fill<int>(L.counters, L.pd.nblk + 4, 0, s));
+  if (last_obj) HZ_TRY(fill<T>(last_obj, 1, T(0), s));
+
+  const EpiArgs<T> epi = make_epi<T>(L, O, E, Pr, theta, sigma, tol, stabilized, last_obj);
+  CentArgs<T> ca;
+  ca.Z = Z;
+  ca.R = R;
+  ca.order = L.ct.order;
+  ca.seg_ch = L.ct.seg_ch;
+  ca.ch_start = L.ct.ch_start;
+  ca.ch_len = L.ct.ch_len;
+  ca.ch_seg = L.ct.ch_seg;
+  ca.acc_in = L.acc_in;
+  ca.Ypart = reinterpret_cast<T*>(L.Ypart);
+  ca.ctrl = L.ctrl;
+  ca.nseg = (int)nseg;
+  ca.cpw = L.cpw;
+  ca.n = L.n;
+  ca.d = L.d;
+  ca.K = L.K;
+  ca.scale_o = L.scale_o;
+
+  YRedArgs<T> ya;
+  ya.Ypart = reinterpret_cast<const T*>(L.Ypart);
+  ya.Y = reinterpret_cast<T*>(L.Y);
+  ya.YT = reinterpret_cast<T*>(L.YT);
+  ya.G = L.G;
+  ya.K = L.K;
+  ya.d = L.d;
+  ya.Kp = L.Kp;
+  ya.counter = L.counters + L.pd.nblk;
+  ya.epi = epi;
+  const int ygrid = (int)std::max<long long>(1, std::min<long long>(ceil_div((long long)L.K * L.d, kBlock), 64));
+
+  RowArgs<T> ra;
+  ra.Z = Z;
+  ra.YT = reinterpret_cast<const T*>(L.YT);
+  ra.R = R;
+  ra.pen = reinterpret_cast<const T*>(L.pen);
+  ra.order = L.ct.order;
+  ra.seg_ch = L.ct.seg_ch;
+  ra.ch_start = L.ct.ch_start;
+  ra.ch_len = L.ct.ch_len;
+  ra.ch_seg = L.ct.ch_seg;
+  ra.acc_out = L.acc_out;
+  ra.term = T(-2.0 / sigma);
+  ra.n = L.n;
+  ra.d = L.d;
+  ra.K = L.K;
+  ra.Kp = L.Kp;
+  ra.vec_store = (L.K % 4 == 0) ? 1 : 0;
+  ra.scale_o = L.scale_o;
+  ra.scale_obj = L.scale_obj;
+  ra.obj_out = nullptr;
+  ra.epi = epi;
+
+  const int check_every = 4;
+  for (int it = 0; it < max_iter; ++it) {
+    if (it > 0 && it % check_every == 0) {
+      int done = 0;
+      HZ_TRY(read_done(L.ctrl, s, &done));
+      if (done) break;
+    }
+    HZ_TRY(launch_block_order(L.pd, seed + (uint32_t)it, cats, L.pb, L.ct, L.nch_tmp, &L.ctrl->done,
+                              s));
+    HZ_TRY(launch_centroid<T>(L, ca, L.G, s));
+    hipLaunchKernelGGL(yreduce_kernel<T>, dim3(ygrid), dim3(kBlock), 0, s, ya);
+    HZ_LAUNCHED("yreduce_kernel");
+    for (int j = 0; j < L.pd.nblk; ++j) {
+      const int rows_j = std::min(L.bs, L.n - j * L.bs);
+      const int grid = (int)(ceil_div(rows_j, L.chunk) + L.B);
+      ra.seg_base = j * L.B;
+      ra.block = j;
+      ra.counter = L.counters + j;
+      HZ_TRY(launch_row<T>(L, ra, grid, s));
+    }
+  }
+  Ctrl h;
+  HZ_HIP(hipMemcpyAsync(&h, L.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s));
+  HZ_HIP(hipStreamSynchronize(s));
+  if (n_iter) *n_iter = h.n_obj;
+  return HZ_OK;
+}
+
+template <typename T, int DT>
+static int launch_apply_dt(const Layout& L, const ApplyArgs<T>& a, int grid, size_t lds,
+                           hipStream_t s) {
+  auto kern = apply_kernel<T, DT>;
+  if (lds > 65536)
+    HZ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), lds, s, a);
+  HZ_LAUNCHED("apply_kernel");
+  return HZ_OK;
+}
+
+template <typename T>
+static int correction_impl(const hz_dims* dm, const T* X, const T* R, const T* O, const T* E,
+                           const T* Nb, const int* cat_offsets, const int* cell_indices,
+                           int method, CorrParams<T> p, T* lam_out, T* Zhat, T* Znorm, void* ws,
+                           size_t ws_bytes, hipStream_t s) {
+  Layout L;
+  HZ_TRY(make_layout(dm, nullptr, L));
+  HZ_TRY(check_ws(L, ws, ws_bytes));
+  HZ_TRY(make_layout(dm, ws, L));
+  HZ_RET_IF(method < HZ_CORR_BATCHED || method > HZ_CORR_ORIGINAL, HZ_ERR_ARG,
+            "unknown correction method %d", method);
+  T* lam = lam_out ? lam_out : reinterpret_cast<T*>(L.lam);
+  const int BK = L.B * L.K;
+  hipLaunchKernelGGL(lambda_kernel<T>, dim3((int)ceil_div(BK, 256)), dim3(256), 0, s, E, O, Nb,
+                     lam, L.B, L.K, p);
+  HZ_LAUNCHED("lambda_kernel");
+  // static chunk tables (cells grouped by batch)
+  HZ_TRY(launch_static_chunks(L.n, L.B, L.chunk_p, cat_offsets, cell_indices, L.sp, L.nch_tmp_p, s));
+  HZ_TRY(launch_static_chunks(L.n, L.B, L.chunk, cat_offsets, cell_indices, L.st, L.nch_tmp_s, s));
+  hipLaunchKernelGGL(ctrl_reset_kernel, dim3(1), dim3(64), 0, s, L.ctrl_static);
+  HZ_LAUNCHED("ctrl_reset");
+  HZ_TRY(fill<long long>(L.rsum, BK, 0, s));
+  // Phi partials: one workgroup per phi chunk
+  CentArgs<T> ca;
+  ca.Z = X;
+  ca.R = R;
+  ca.order = L.sp.order;
+  ca.seg_ch = L.sp.seg_ch;
+  ca.ch_start = L.sp.ch_start;
+  ca.ch_len = L.sp.ch_len;
+  ca.ch_seg = L.sp.ch_seg;
+  ca.acc_in = L.rsum;
+  ca.Ypart = reinterpret_cast<T*>(L.phipart);
+  ca.ctrl = L.ctrl_static;
+  ca.nseg = L.B;
+  ca.cpw = 1;
+  ca.n = L.n;
+  ca.d = L.d;
+  ca.K = L.K;
+  ca.scale_o = L.scale_o;
+  HZ_TRY(launch_centroid<T>(L, ca, (int)L.spd.maxch, s));
+  {
+    long long tot = (long long)BK * L.d;
+    int g = (int)std::max<long long>(1, std::min<long long>(ceil_div(tot, 256), 4096));
+    hipLaunchKernelGGL(phi_reduce_kernel<T>, dim3(g), dim3(256), 0, s,
+                       reinterpret_cast<const T*>(L.phipart), L.sp.seg_ch, L.Phi, L.B, L.K, L.d);
+    HZ_LAUNCHED("phi_reduce_kernel");
+  }
+  {
+    WArgs<T> wa;
+    wa.Phi = L.Phi;
+    wa.rsum = L.rsum;
+    wa.inv_scale = 1.0 / L.scale_o;
+    wa.O = O;
+    wa.lam = lam;
+    wa.Wt = reinterpret_cast<T*>(L.Wt);
+    wa.B = L.B;
+    wa.K = L.K;
+    wa.d = L.d;
+    wa.method = method;
+    size_t lds = (size_t)(3 * L.B + L.d) * sizeof(double);
+    if (method == HZ_CORR_ORIGINAL && L.B + 1 <= kCholMax)
+      lds += (size_t)((L.B + 1) * (L.B + 1) + (L.B + 1) * L.d) * sizeof(double);
+    HZ_RET_IF(lds > (size_t)device_lds_limit(), HZ_ERR_UNSUPPORTED,
+              "correction solve needs %zu bytes of LDS", lds);
+    if (lds > 65536)
+      HZ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(w_kernel<T>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(w_kernel<T>, dim3(L.K), dim3(kBlock), lds, s, wa);
+    HZ_LAUNCHED("w_kernel");
+  }
+  ApplyArgs<T> aa;
+  aa.X = X;
+  aa.R = R;
+  aa.Wt = reinterpret_cast<const T*>(L.Wt);
+  aa.Zhat = Zhat;
+  aa.Znorm = Znorm;
+  aa.order = L.st.order;
+  aa.seg_ch = L.st.seg_ch;
+  aa.ch_start = L.st.ch_start;
+  aa.ch_len = L.st.ch_len;
+  aa.ch_seg = L.st.ch_seg;
+  aa.nb = L.B;
+  aa.n = L.n;
+  aa.d = L.d;
+  aa.K = L.K;
+  const size_t lds = (size_t)L.K * L.Dp * sizeof(T) + (size_t)L.K * kZtStride * sizeof(T);
+  HZ_RET_IF(lds > (size_t)device_lds_limit(), HZ_ERR_UNSUPPORTED,
+            "apply kernel needs %zu bytes of LDS", lds);
+  const int grid = (int)L.sd.maxch;
+  switch (L.DT) {
+    case 2: return launch_apply_dt<T, 2>(L, aa, grid, lds, s);
+    case 4: return launch_apply_dt<T, 4>(L, aa, grid, lds, s);
+    default: return launch_apply_dt<T, 8>(L, aa, grid, lds, s);
+  }
+}
+
+// ---------------------------------------------------------------- small kernels
+
+template <typename T>
+__global__ void penalty_kernel(const T* E, const T* O, const T* theta, T* pen, int B, int K,
+                               int stabilized) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < B * K; i += gridDim.x * blockDim.x)
+    pen[i] = penalty_value(E[i], O[i], theta[i / K], stabilized);
+}
+
+// One wave per row.
+template <typename T>
+__global__ void fused_pen_norm_kernel(const T* sim, const T* pen, const int* cats, const int* idx,
+                                      T* out, T term, int rows, int K) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w0 = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int nw = (gridDim.x * blockDim.x) / kWave;
+  for (int r = w0; r < rows; r += nw) {
+    const long long srow = idx[r];
+    const int b = cats[r];
+    T s = T(0);
+    for (int k = lane; k < K; k += kWave) {
+      T v = dexp(term * (T(1) - sim[srow * K + k])) * pen[(long long)b * K + k];
+      out[(long long)r * K + k] = v;
+      s += v;
+    }
+    s = wave_sum(s);
+    const T inv = T(1) / s;
+    for (int k = lane; k < K; k += kWave) out[(long long)r * K + k] *= inv;
+  }
+}
+
+constexpr int kObjGrid = 256;
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void objective_terms_kernel(const T* R, const T* sim, int n,
+                                                                int K, double* part) {
+  __shared__ double red[kBlock / kWave];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w0 = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int nw = (gridDim.x * blockDim.x) / kWave;
+  double km = 0.0, ent = 0.0;
+  for (int r = w0; r < n; r += nw) {
+    const T* rr = R + (long long)r * K;
+    const T* sr = sim + (long long)r * K;
+    T rs = T(0);
+    for (int k = lane; k < K; k += kWave) {
+      rs += rr[k];
+      km += (double)(rr[k] * T(2) * (T(1) - sr[k]));
+    }
+    rs = wave_sum(rs);
+    const T inv = T(1) / rs;
+    for (int k = lane; k < K; k += kWave) {
+      T x = rr[k] * inv;
+      ent += (double)(x * dlog(x + T(1e-12)));
+    }
+  }
+  km = block_sum(km, red);
+  ent = block_sum(ent, red);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = km;
+    part[2 * blockIdx.x + 1] = ent;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void objective_final_kernel(const double* part, int G,
+                                                                const T* O, const T* E,
+                                                                const T* theta, T sigma, int B,
+                                                                int K, int stabilized,
+                                                                double* out) {
+  __shared__ double red[kBlock / kWave];
+  double div = 0.0;
+  for (int p = threadIdx.x; p < B * K; p += blockDim.x) {
+    T o = O[p], e = E[p];
+    T numer = stabilized ? (o + e + T(1)) : (o + T(1));
+    div += (double)(theta[p / K] * o * dlog(numer / (e + T(1))));
+  }
+  div = block_sum(div, red);
+  if (threadIdx.x == 0) {
+    double km = 0.0, ent = 0.0;
+    for (int g = 0; g < G; ++g) {
+      km += part[2 * g];
+      ent += part[2 * g + 1];
+    }
+    out[0] = (double)T(km + (double)sigma * ent + (double)sigma * div);
+  }
+}
+
+// k-means workspace: key order buffers + YT/half + partials.
+struct KmLayout {
+  PermDims p;
+  PermBufs b;
+  ChunkTab t;
+  void* YT;
+  void* half;
+  long long* part;
+  int G, tiles_per_wg, Kp, CT;
+  size_t bytes;
+};
+
+static int km_layout(int dtype, long long n, int d, int k, void* base, KmLayout& L) {
+  HZ_RET_IF(dtype != HZ_F32 && dtype != HZ_F64, HZ_ERR_ARG, "dtype must be 4 or 8");
+  HZ_RET_IF(n < 1 || d < 1 || k < 1 || k > n, HZ_ERR_ARG, "bad k-means shape");
+  HZ_RET_IF(n > 0x7fffffffLL, HZ_ERR_UNSUPPORTED, "too many cells");
+  L.CT = pick_ct(k);
+  HZ_RET_IF(!L.CT, HZ_ERR_UNSUPPORTED, "k=%d > 256 is not supported", k);
+  L.Kp = 16 * L.CT;
+  L.p = perm_dims((int)n, (int)n, 1, 64);
+  long long tiles = ceil_div(n, kTileRows);
+  L.G = (int)std::min<long long>(tiles, 512);
+  L.tiles_per_wg = (int)ceil_div(tiles, L.G);
+  L.G = (int)ceil_div(tiles, L.tiles_per_wg);
+  Carve c(base);
+  carve_perm(c, L.p, L.b, L.t);
+  L.YT = c.take<char>((size_t)d * L.Kp * dtype);
+  L.half = c.take<char>((size_t)L.Kp * dtype);
+  L.part = c.take<long long>((size_t)L.G * ((size_t)k * d + k));
+  L.bytes = align_up(c.off, 256);
+  return HZ_OK;
+}
+
+template <typename T>
+static int kmeans_impl(const T* Z, long long n, int d, int k, uint32_t seed, int max_iter, T* Y,
+                       void* ws, size_t ws_bytes, hipStream_t s) {
+  KmLayout L;
+  HZ_TRY(km_layout(sizeof(T), n, d, k, nullptr, L));
+  HZ_RET_IF(!ws || ws_bytes < L.bytes, HZ_ERR_WORKSPACE, "workspace too small: %zu < %zu",
+            ws_bytes, L.bytes);
+  HZ_TRY(km_layout(sizeof(T), n, d, k, ws, L));
+  HZ_TRY(launch_key_order(L.p, seed, L.b, nullptr, s));
+  hipLaunchKernelGGL(gather_rows_kernel<T>, dim3((int)std::max<long long>(1, ceil_div((long long)k * d, 256))),
+                     dim3(256), 0, s, Z, L.b.perm, Y, k, d);
+  HZ_LAUNCHED("gather_rows");
+  const size_t lds = (size_t)d * L.Kp * sizeof(T) + (size_t)d * kZtStride * sizeof(T) +
+                     ((size_t)k * d + k) * sizeof(long long);
+  HZ_RET_IF(lds > (size_t)device_lds_limit(), HZ_ERR_UNSUPPORTED,
+            "k-means needs %zu bytes of LDS", lds);
+  KmArgs<T> a;
+  a.Z = Z;
+  a.YT = reinterpret_cast<const T*>(L.YT);
+  a.half = reinterpret_cast<const T*>(L.half);
+  a.part = L.part;
+  a.n = (int)n;
+  a.d = d;
+  a.K = k;
+  a.tiles_per_wg = L.tiles_per_wg;
+  a.scale = std::ldexp(1.0, 32);
+  for (int it = 0; it < max_iter; ++it) {
+    hipLaunchKernelGGL(kmeans_prep_kernel<T>, dim3(1), dim3(256), 0, s, Y,
+                       reinterpret_cast<T*>(L.YT), reinterpret_cast<T*>(L.half), k, d, L.Kp);
+    HZ_LAUNCHED("kmeans_prep");
+    const void* fn;
+    switch (L.CT) {
+      case 4: fn = reinterpret_cast<const void*>(kmeans_assign_kernel<T, 4>); break;
+      case 8: fn = reinterpret_cast<const void*>(kmeans_assign_kernel<T, 8>); break;
+      default: fn = reinterpret_cast<const void*>(kmeans_assign_kernel<T, 16>); break;
+    }
+    if (lds > 65536)
+      HZ_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    switch (L.CT) {
+      case 4: hipLaunchKernelGGL((kmeans_assign_kernel<T, 4>), dim3(L.G), dim3(kBlock), lds, s, a); break;
+      case 8: hipLaunchKernelGGL((kmeans_assign_kernel<T, 8>), dim3(L.G), dim3(kBlock), lds, s, a); break;
+      default: hipLaunchKernelGGL((kmeans_assign_kernel<T, 16>), dim3(L.G), dim3(kBlock), lds, s, a); break;
+    }
+    HZ_LAUNCHED("kmeans_assign");
+    hipLaunchKernelGGL(kmeans_update_kernel<T>, dim3((int)std::max<long long>(1, ceil_div((long long)k * d, 256))),
+                       dim3(256), 0, s, L.part, L.G, Y, k, d, std::ldexp(1.0, -32));
+    HZ_LAUNCHED("kmeans_update");
+  }
+  return HZ_OK;
+}
+
+}  // namespace hz
+
+using namespace hz;
+
+// ================================================================ C ABI
+
+extern "C" {
+
+int hz_version(void) { return 100; }
+
+const char* hz_last_error(void) { return hz::g_err; }
+
+size_t hz_workspace_bytes(const hz_dims* dims) {
+  Layout L;
+  if (make_layout(dims, nullptr, L) != HZ_OK) return 0;
+  return L.bytes;
+}
+
+int hz_l2_normalize(int dtype, const void* src, void* dst, long long n_rows, int n_cols,
+                    void* stream) {
+  HZ_RET_IF(n_rows < 0 || n_cols < 1, HZ_ERR_ARG, "bad shape %lld x %d", n_rows, n_cols);
+  if (n_rows == 0) return HZ_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int grid = (int)std::min<long long>(ceil_div(n_rows, 4), (long long)device_cu_count() * 16);
+  if (dtype == HZ_F32)
+    hipLaunchKernelGGL(l2_rows_kernel<float>, dim3(grid), dim3(256), 0, s,
+                       static_cast<const float*>(src), static_cast<float*>(dst), n_rows, n_cols);
+  else if (dtype == HZ_F64)
+    hipLaunchKernelGGL(l2_rows_kernel<double>, dim3(grid), dim3(256), 0, s,
+                       static_cast<const double*>(src), static_cast<double*>(dst), n_rows, n_cols);
+  else
+    HZ_RET_IF(true, HZ_ERR_ARG, "dtype must be 4 or 8");
+  HZ_LAUNCHED("l2_rows_kernel");
+  return HZ_OK;
+}
+
+size_t hz_block_order_workspace_bytes(int n_cells) {
+  if (n_cells < 1) return 0;
+  PermDims p = perm_dims(n_cells, n_cells, 1, 64);
+  Carve c(nullptr);
+  PermBufs b;
+  ChunkTab t;
+  carve_perm(c, p, b, t);
+  return align_up(c.off, 256);
+}
+
+int hz_block_order(int n_cells, uint32_t seed, int32_t* order, void* ws, size_t ws_bytes,
+                   void* stream) {
+  HZ_RET_IF(n_cells < 1, HZ_ERR_ARG, "n_cells must be >= 1");
+  HZ_RET_IF(!order, HZ_ERR_ARG, "order is NULL");
+  size_t need = hz_block_order_workspace_bytes(n_cells);
+  HZ_RET_IF(!ws || ws_bytes < need, HZ_ERR_WORKSPACE, "workspace too small: %zu < %zu",
+            ws_bytes, need);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  PermDims p = perm_dims(n_cells, n_cells, 1, 64);
+  Carve c(ws);
+  PermBufs b;
+  ChunkTab t;
+  carve_perm(c, p, b, t);
+  b.perm = order;
+  return launch_key_order(p, seed, b, nullptr, s);
+}
+
+int hz_init_clusters(const hz_dims* dims, const void* Z_norm, const void* Y0, const int32_t* cats,
+                     const int32_t* cat_offsets, const int32_t* cell_indices, const void* Pr_b,
+                     const void* theta, double sigma, int stabilized, void* R, void* E, void* O,
+                     void* obj_out, void* ws, size_t ws_bytes, void* stream) {
+  HZ_RET_IF(!dims, HZ_ERR_ARG, "dims is NULL");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dims->dtype == HZ_F32)
+    return init_impl<float>(dims, (const float*)Z_norm, (const float*)Y0, cats, cat_offsets,
+                            cell_indices, (const float*)Pr_b, (const float*)theta, sigma,
+                            stabilized, (float*)R, (float*)E, (float*)O, (float*)obj_out, ws,
+                            ws_bytes, s);
+  if (dims->dtype == HZ_F64)
+    return init_impl<double>(dims, (const double*)Z_norm, (const double*)Y0, cats, cat_offsets,
+                             cell_indices, (const double*)Pr_b, (const double*)theta, sigma,
+                             stabilized, (double*)R, (double*)E, (double*)O, (double*)obj_out, ws,
+                             ws_bytes, s);
+  HZ_RET_IF(true, HZ_ERR_ARG, "dtype must be 4 or 8");
+}
+
+int hz_clustering_loop(const hz_dims* dims, const void* Z_norm, void* R, void* E, void* O,
+                       const void* Pr_b, const int32_t* cats, const void* theta, double sigma,
+                       double tol, int max_iter, uint32_t seed, int stabilized, void* last_obj,
+                       int* n_iter, void* ws, size_t ws_bytes, void* stream) {
+  HZ_RET_IF(!dims, HZ_ERR_ARG, "dims is NULL");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dims->dtype == HZ_F32)
+    return clustering_impl<float>(dims, (const float*)Z_norm, (float*)R, (float*)E, (float*)O,
+                                  (const float*)Pr_b, cats, (const float*)theta, sigma, tol,
+                                  max_iter, seed, stabilized, (float*)last_obj, n_iter, ws,
+                                  ws_bytes, s);
+  if (dims->dtype == HZ_F64)
+    return clustering_impl<double>(dims, (const double*)Z_norm, (double*)R, (double*)E,
+                                   (double*)O, (const double*)Pr_b, cats, (const double*)theta,
+                                   sigma, tol, max_iter, seed, stabilized, (double*)last_obj,
+                                   n_iter, ws, ws_bytes, s);
+  HZ_RET_IF(true, HZ_ERR_ARG, "dtype must be 4 or 8");
+}
+
+int hz_correction(const hz_dims* dims, const void* X, const void* R, const void* O, const void* E,
+                  const void* N_b, const int32_t* cat_offsets, const int32_t* cell_indices,
+                  int method, int dynamic_lambda, double alpha, int has_threshold,
+                  double threshold, double ridge_lambda, void* lambda_kb_out, void* Z_hat,
+                  void* Z_norm, void* ws, size_t ws_bytes, void* stream) {
+  HZ_RET_IF(!dims, HZ_ERR_ARG, "dims is NULL");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dims->dtype == HZ_F32) {
+    CorrParams<float> p{dynamic_lambda, (float)alpha, has_threshold, (float)threshold,
+                        (float)ridge_lambda, method};
+    return correction_impl<float>(dims, (const float*)X, (const float*)R, (const float*)O,
+                                  (const float*)E, (const float*)N_b, cat_offsets, cell_indices,
+                                  method, p, (float*)lambda_kb_out, (float*)Z_hat, (float*)Z_norm,
+                                  ws, ws_bytes, s);
+  }
+  if (dims->dtype == HZ_F64) {
+    CorrParams<double> p{dynamic_lambda, alpha, has_threshold, threshold, ridge_lambda, method};
+    return correction_impl<double>(dims, (const double*)X, (const double*)R, (const double*)O,
+                                   (const double*)E, (const double*)N_b, cat_offsets,
+                                   cell_indices, method, p, (double*)lambda_kb_out,
+                                   (double*)Z_hat, (double*)Z_norm, ws, ws_bytes, s);
+  }
+  HZ_RET_IF(true, HZ_ERR_ARG, "dtype must be 4 or 8");
+}
+
+size_t hz_objective_workspace_bytes(void) { return (size_t)(2 * kObjGrid + 8) * sizeof(double); }
+
+int hz_objective(int dtype, const void* R, const void* sim, const void* O, const void* E,
+                 const void* theta, double sigma, int n_cells, int n_clusters, int n_batches,
+                 int stabilized, double* obj_host, void* ws, size_t ws_bytes, void* stream) {
+  HZ_RET_IF(n_cells < 0 || n_clusters < 1 || n_batches < 1, HZ_ERR_ARG, "bad shape");
+  HZ_RET_IF(!ws || ws_bytes < hz_objective_workspace_bytes(), HZ_ERR_WORKSPACE,
+            "workspace too small");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  double* part = static_cast<double*>(ws);
+  double* out = part + 2 * kObjGrid;
+  if (dtype == HZ_F32) {
+    hipLaunchKernelGGL(objective_terms_kernel<float>, dim3(kObjGrid), dim3(kBlock), 0, s,
+                       (const float*)R, (const float*)sim, n_cells, n_clusters, part);
+    HZ_LAUNCHED("objective_terms");
+    hipLaunchKernelGGL(objective_final_kernel<float>, dim3(1), dim3(kBlock), 0, s, part, kObjGrid,
+                       (const float*)O, (const float*)E, (const float*)theta, (float)sigma,
+                       n_batches, n_clusters, stabilized, out);
+  } else if (dtype == HZ_F64) {
+    hipLaunchKernelGGL(objective_terms_kernel<double>, dim3(kObjGrid), dim3(kBlock), 0, s,
+                       (const double*)R, (const double*)sim, n_cells, n_clusters, part);
+    HZ_LAUNCHED("objective_terms");
+    hipLaunchKernelGGL(objective_final_kernel<double>, dim3(1), dim3(kBlock), 0, s, part,
+                       kObjGrid, (const double*)O, (const double*)E, (const double*)theta, sigma,
+                       n_batches, n_clusters, stabilized, out);
+  } else {
+    HZ_RET_IF(true, HZ_ERR_ARG, "dtype must be 4 or 8");
+  }
+  HZ_LAUNCHED("objective_final");
+  HZ_HIP(hipMemcpyAsync(obj_host, out, sizeof(double), hipMemcpyDeviceToHost, s));
+  HZ_HIP(hipStreamSynchronize(s));
+  return HZ_OK;
+}
+
+int hz_penalty(int dtype, const void* E, const void* O, const void* theta, void* pen,
+               int n_batches, int n_clusters, int stabilized, void* stream) {
+  HZ_RET_IF(n_batches < 1 || n_clusters < 1, HZ_ERR_ARG, "bad shape");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int grid = (int)ceil_div((long long)n_batches * n_clusters, 256);
+  if (dtype == HZ_F32)
+    hipLaunchKernelGGL(penalty_kernel<float>, dim3(grid), dim3(256), 0, s, (const float*)E,
+                       (const float*)O, (const float*)theta, (float*)pen, n_batches, n_clusters,
+                       stabilized);
+  else if (dtype == HZ_F64)
+    hipLaunchKernelGGL(penalty_kernel<double>, dim3(grid), dim3(256), 0, s, (const double*)E,
+                       (const double*)O, (const double*)theta, (double*)pen, n_batches,
+                       n_clusters, stabilized);
+  else
+    HZ_RET_IF(true, HZ_ERR_ARG, "dtype must be 4 or 8");
+  HZ_LAUNCHED("penalty_kernel");
+  return HZ_OK;
+}
+
+int hz_fused_pen_norm(int dtype, const void* sim, const void* pen, const int32_t* cats,
+                      const int32_t* idx, void* R_out, double term, int n_rows, int n_cols,
+                      void* stream) {
+  HZ_RET_IF(n_rows < 0 || n_cols < 1, HZ_ERR_ARG, "bad shape");
+  if (n_rows == 0) return HZ_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int grid = (int)std::min<long long>(ceil_div(n_rows, 4), 4096);
+  if (dtype == HZ_F32)
+    hipLaunchKernelGGL(fused_pen_norm_kernel<float>, dim3(grid), dim3(256), 0, s,
+                       (const float*)sim, (const float*)pen, cats, idx, (float*)R_out,
+                       (float)term, n_rows, n_cols);
+  else if (dtype == HZ_F64)
+    hipLaunchKernelGGL(fused_pen_norm_kernel<double>, dim3(grid), dim3(256), 0, s,
+                       (const double*)sim, (const double*)pen, cats, idx, (double*)R_out, term,
+                       n_rows, n_cols);
+  else
+    HZ_RET_IF(true, HZ_ERR_ARG, "dtype must be 4 or 8");
+  HZ_LAUNCHED("fused_pen_norm_kernel");
+  return HZ_OK;
+}
+
+int hz_lambda_kb(int dtype, const void* E, const void* O, const void* N_b, int n_batches,
+                 int n_clusters, int dynamic_lambda, double alpha, int has_threshold,
+                 double threshold, double ridge_lambda, void* out, void* stream) {
+  HZ_RET_IF(n_batches < 1 || n_clusters < 1, HZ_ERR_ARG, "bad shape");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int grid = (int)ceil_div((long long)n_batches * n_clusters, 256);
+  if (dtype == HZ_F32) {
+    CorrParams<float> p{dynamic_lambda, (float)alpha, has_threshold, (float)threshold,
+                        (float)ridge_lambda, 0};
+    hipLaunchKernelGGL(lambda_kernel<float>, dim3(grid), dim3(256), 0, s, (const float*)E,
+                       (const float*)O, (const float*)N_b, (float*)out, n_batches, n_clusters, p);
+  } else if (dtype == HZ_F64) {
+    CorrParams<double> p{dynamic_lambda, alpha, has_threshold, threshold, ridge_lambda, 0};
+    hipLaunchKernelGGL(lambda_kernel<double>, dim3(grid), dim3(256), 0, s, (const double*)E,
+                       (const double*)O, (const double*)N_b, (double*)out, n_batches, n_clusters,
+                       p);
+  } else {
+    HZ_RET_IF(true, HZ_ERR_ARG, "dtype must be 4 or 8");
+  }
+  HZ_LAUNCHED("lambda_kernel");
+  return HZ_OK;
+}
+
+int hz_inv_mats(int dtype, const void* O, const void* lambda_kb, void* inv, int n_batches,
+                int n_clusters, void* stream) {
+  HZ_RET_IF(n_batches < 1 || n_clusters < 1, HZ_ERR_ARG, "bad shape");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  size_t lds = (size_t)2 * n_batches * sizeof(double);
+  HZ_RET_IF(lds > (size_t)device_lds_limit(), HZ_ERR_UNSUPPORTED, "too many batches");
+  if (dtype == HZ_F32)
+    hipLaunchKernelGGL(inv_mat_kernel<float>, dim3(n_clusters), dim3(256), lds, s,
+                       (const float*)O, (const float*)lambda_kb, (float*)inv, n_batches,
+                       n_clusters);
+  else if (dtype == HZ_F64)
+    hipLaunchKernelGGL(inv_mat_kernel<double>, dim3(n_clusters), dim3(256), lds, s,
+                       (const double*)O, (const double*)lambda_kb, (double*)inv, n_batches,
+                       n_clusters);
+  else
+    HZ_RET_IF(true, HZ_ERR_ARG, "dtype must be 4 or 8");
+  HZ_LAUNCHED("inv_mat_kernel");
+  return HZ_OK;
+}
+
+size_t hz_kmeans_workspace_bytes(int dtype, long long n_cells, int n_pcs, int k) {
+  KmLayout L;
+  if (km_layout(dtype, n_cells, n_pcs, k, nullptr, L) != HZ_OK) return 0;
+  return L.bytes;
+}
+
+int hz_kmeans(int dtype, const void* Z_norm, long long n_cells, int n_pcs, int k, uint32_t seed,
+              int max_iter, void* Y_out, void* ws, size_t ws_bytes, void* stream) {
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == HZ_F32)
+    return kmeans_impl<float>((const float*)Z_norm, n_cells, n_pcs, k, seed, max_iter,
+                              (float*)Y_out, ws, ws_bytes, s);
+  if (dtype == HZ_F64)
+    return kmeans_impl<double>((const double*)Z_norm, n_cells, n_pcs, k, seed, max_iter,
+                               (double*)Y_out, ws, ws_bytes, s);
+  HZ_RET_IF(true, HZ_ERR_ARG, "dtype must be 4 or 8");
+}
+
+}  // extern "C"

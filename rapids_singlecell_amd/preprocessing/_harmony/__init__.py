@@ -1,0 +1,251 @@
+"""Harmony driver over the HIP library.
+
+Restates ``harmonize`` of the reference (preprocessing/_harmony/__init__.py:
+41-289): validation, batch bookkeeping, initialisation, the Harmony outer loop
+with clustering + correction, and the convergence test.  All state (Z_norm, R,
+E, O, Y, Z_hat) stays in device memory; per Harmony iteration the host reads
+one scalar (the clustering objective) and, during clustering, a done flag
+every few iterations.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import warnings
+
+import numpy as np
+import torch
+
+from ... import _native
+from ._helper import _create_category_index_mapping, _get_batch_codes, _get_theta_array
+
+_SUPPRESS_PENALTY = 1e30
+COLSUM_ALGOS = ["columns", "atomics", "gemm", "benchmark", None]
+_U32 = 0xFFFFFFFF
+_PRECISIONS = {"native": 0, "fp32": 0, "fp64": 0, "split_bf16": 1}
+
+
+def _stream_handle(device: torch.device) -> ctypes.c_void_p:
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def _ptr(t: torch.Tensor) -> ctypes.c_void_p:
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _to_device(x, device, dtype):
+    if isinstance(x, torch.Tensor):
+        return x.to(device=device, dtype=dtype).contiguous()
+    return torch.as_tensor(np.ascontiguousarray(x), dtype=dtype, device=device)
+
+
+def kmeans_centroids(Z_norm: torch.Tensor, n_clusters: int, random_state: int,
+                     max_iter: int = 25) -> torch.Tensor:
+    """Initial centroids from the library's seeded Lloyd k-means on the GPU."""
+    if not Z_norm.is_cuda:
+        raise TypeError("kmeans_centroids needs a device tensor")
+    lib = _native.load()
+    n, d = Z_norm.shape
+    dsz = Z_norm.element_size()
+    dev = Z_norm.device
+    nbytes = lib.hz_kmeans_workspace_bytes(dsz, n, d, n_clusters)
+    if nbytes == 0:
+        _native.check(-4, "hz_kmeans_workspace_bytes")
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    Y = torch.empty((n_clusters, d), dtype=Z_norm.dtype, device=dev)
+    rc = lib.hz_kmeans(dsz, _ptr(Z_norm), n, d, n_clusters, random_state & _U32, max_iter,
+                       _ptr(Y), _ptr(ws), nbytes, _stream_handle(dev))
+    _native.check(rc, "hz_kmeans")
+    return Y
+
+
+def l2_normalize(X: torch.Tensor) -> torch.Tensor:
+    if not X.is_cuda:
+        raise TypeError("l2_normalize needs a device tensor")
+    X = X.contiguous()
+    lib = _native.load()
+    out = torch.empty_like(X)
+    rc = lib.hz_l2_normalize(X.element_size(), _ptr(X), _ptr(out), X.shape[0], X.shape[1],
+                             _stream_handle(X.device))
+    _native.check(rc, "hz_l2_normalize")
+    return out
+
+
+def harmonize(
+    Z,
+    batch_mat,
+    batch_key,
+    *,
+    n_clusters: int | None = None,
+    max_iter_harmony: int = 10,
+    max_iter_clustering: int = 200,
+    tol_harmony: float = 1e-4,
+    tol_clustering: float = 1e-5,
+    ridge_lambda: float = 1.0,
+    sigma: float = 0.1,
+    block_proportion: float = 0.05,
+    theta=2.0,
+    tau: int = 0,
+    correction_method: str | None = None,
+    colsum_algo: str | None = None,
+    random_state: int = 0,
+    stabilized_penalty: bool = True,
+    dynamic_lambda: bool = True,
+    alpha: float = 0.2,
+    batch_prune_threshold: float | None = 1e-5,
+    verbose: bool = False,
+    init_centroids=None,
+    precision: str = "native",
+    record: dict | None = None,
+) -> torch.Tensor:
+    """Integrate data using the Harmony algorithm (device tensors in and out).
+
+    Parameters and errors follow the reference ``harmonize``.  Extensions:
+    ``init_centroids`` (K x d) replaces the k-means initialiser (parity runs
+    inject the same centroids into the oracle); ``precision`` selects the GEMM
+    arithmetic; ``record`` receives objectives and clustering iteration counts.
+    ``colsum_algo`` is validated and otherwise ignored: column sums are fused
+    into the row kernel.
+    """
+    if not isinstance(Z, torch.Tensor):
+        raise TypeError("harmonize expects a device tensor; use harmony_integrate for AnnData")
+    if Z.dtype not in (torch.float32, torch.float64):
+        raise TypeError(f"unsupported dtype {Z.dtype}")
+    dev = Z.device
+    Z = Z.contiguous()
+    n_cells, n_pcs = Z.shape
+    dsz = Z.element_size()
+    np_dtype = np.float32 if Z.dtype == torch.float32 else np.float64
+    if precision not in _PRECISIONS:
+        raise ValueError(f"precision must be one of {sorted(_PRECISIONS)}, got {precision!r}")
+    prec = _PRECISIONS[precision]
+    if prec == 1 and Z.dtype != torch.float32:
+        raise ValueError("precision='split_bf16' needs float32 data")
+
+    # batch bookkeeping (_harmony/__init__.py:146-152)
+    batch_codes = _get_batch_codes(batch_mat, batch_key)
+    n_batches = batch_codes.cat.categories.size
+    N_b_np = np.asarray(batch_codes.value_counts(sort=False).values, dtype=np_dtype)
+    Pr_b_np = (N_b_np.reshape(-1, 1) / len(batch_codes)).astype(np_dtype)
+    cats_np = np.asarray(batch_codes.cat.codes.values, dtype=np.int32)
+    if n_cells and cats_np.min() < 0:
+        raise ValueError(f"batch key {batch_key!r} has missing values")
+    offsets_np, cell_idx_np = _create_category_index_mapping(cats_np, n_batches)
+
+    # parameters (_harmony/__init__.py:155-200)
+    if max_iter_harmony < 1:
+        raise ValueError("max_iter_harmony must be >= 1")
+    if n_clusters is None:
+        n_clusters = int(min(100, n_cells / 30))
+        n_clusters = max(n_clusters, 2)
+    assert colsum_algo in COLSUM_ALGOS
+    theta_np = _get_theta_array(theta, n_batches, np_dtype)
+    if tau > 0:
+        theta_np = theta_np * (1 - np.exp(-N_b_np / (n_clusters * tau)) ** 2)
+    theta_np = np.ascontiguousarray(theta_np.astype(np_dtype).ravel())
+    assert block_proportion > 0 and block_proportion <= 1
+    if dynamic_lambda:
+        if not np.isfinite(alpha) or alpha <= 0:
+            raise ValueError(
+                f"alpha must be a finite positive number when dynamic_lambda=True, got {alpha}."
+            )
+        if batch_prune_threshold is not None and not (0 <= batch_prune_threshold <= 1):
+            raise ValueError(
+                f"batch_prune_threshold must be in [0, 1] or None, got {batch_prune_threshold}."
+            )
+    if correction_method is not None and correction_method not in {"fast", "original", "batched"}:
+        raise ValueError("correction_method must be 'fast', 'original', or 'batched'.")
+    if correction_method is None:
+        nb1 = n_batches + 1
+        inv_bytes = n_clusters * nb1 * nb1 * dsz
+        correction_method = "batched" if inv_bytes <= (1 << 30) else "fast"
+    block_size = int(n_cells * block_proportion)
+    if block_size < 1:
+        raise ValueError(
+            f"block_proportion={block_proportion} gives an empty block for {n_cells} cells"
+        )
+
+    if not Z.is_cuda:
+        raise TypeError("harmonize needs Z on a HIP device (torch 'cuda' tensor)")
+    Z_norm = l2_normalize(Z)
+    tdt = Z.dtype
+    cats = torch.as_tensor(cats_np, device=dev)
+    cat_offsets = torch.as_tensor(offsets_np, device=dev)
+    cell_indices = torch.as_tensor(cell_idx_np, device=dev)
+    N_b = torch.as_tensor(N_b_np, device=dev)
+    Pr_b = torch.as_tensor(Pr_b_np.ravel(), device=dev)
+    theta_t = torch.as_tensor(theta_np, device=dev)
+
+    lib = _native.load()
+    dims = _native.dims(n_cells, n_pcs, n_clusters, n_batches, block_size, dsz, prec)
+    nbytes = lib.hz_workspace_bytes(ctypes.byref(dims))
+    if nbytes == 0:
+        msg = lib.hz_last_error().decode()
+        raise ValueError(f"unsupported Harmony shape: {msg}")
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    stream = _stream_handle(dev)
+
+    # initialisation (_harmony/__init__.py:206-217, 292-353)
+    if init_centroids is None:
+        Y0 = kmeans_centroids(Z_norm, n_clusters, random_state)
+    else:
+        Y0 = _to_device(init_centroids, dev, tdt)
+        if tuple(Y0.shape) != (n_clusters, n_pcs):
+            raise ValueError(f"init_centroids must have shape {(n_clusters, n_pcs)}")
+    R = torch.empty((n_cells, n_clusters), dtype=tdt, device=dev)
+    E = torch.empty((n_batches, n_clusters), dtype=tdt, device=dev)
+    O = torch.empty((n_batches, n_clusters), dtype=tdt, device=dev)
+    obj = torch.empty(1, dtype=tdt, device=dev)
+    rc = lib.hz_init_clusters(ctypes.byref(dims), _ptr(Z_norm), _ptr(Y0), _ptr(cats),
+                              _ptr(cat_offsets), _ptr(cell_indices), _ptr(Pr_b), _ptr(theta_t),
+                              float(sigma), int(bool(stabilized_penalty)), _ptr(R), _ptr(E),
+                              _ptr(O), _ptr(obj), _ptr(ws), nbytes, stream)
+    _native.check(rc, "hz_init_clusters")
+    objectives_harmony = [float(obj.item())]
+
+    method = _native.HZ_CORR[correction_method]
+    has_thr = batch_prune_threshold is not None
+    thr = float(batch_prune_threshold) if has_thr else 0.0
+    Z_hat = torch.empty_like(Z)
+    lam = torch.empty((n_batches, n_clusters), dtype=tdt, device=dev)
+    last_obj = torch.zeros(1, dtype=tdt, device=dev)
+    n_iter = ctypes.c_int(0)
+    clustering_iters = []
+    is_converged = False
+    for i in range(max_iter_harmony):
+        seed = (random_state + i * 1000003) & _U32
+        rc = lib.hz_clustering_loop(ctypes.byref(dims), _ptr(Z_norm), _ptr(R), _ptr(E), _ptr(O),
+                                    _ptr(Pr_b), _ptr(cats), _ptr(theta_t), float(sigma),
+                                    float(tol_clustering), int(max_iter_clustering), seed,
+                                    int(bool(stabilized_penalty)), _ptr(last_obj),
+                                    ctypes.byref(n_iter), _ptr(ws), nbytes, stream)
+        _native.check(rc, "hz_clustering_loop")
+        clustering_iters.append(n_iter.value)
+        objectives_harmony.append(float(last_obj.item()))
+        rc = lib.hz_correction(ctypes.byref(dims), _ptr(Z), _ptr(R), _ptr(O), _ptr(E), _ptr(N_b),
+                               _ptr(cat_offsets), _ptr(cell_indices), method,
+                               int(bool(dynamic_lambda)), float(alpha), int(has_thr), thr,
+                               float(ridge_lambda), _ptr(lam), _ptr(Z_hat), _ptr(Z_norm),
+                               _ptr(ws), nbytes, stream)
+        _native.check(rc, "hz_correction")
+        if _is_convergent_harmony(objectives_harmony, tol=tol_harmony):
+            is_converged = True
+            if verbose:
+                print(f"Harmony converged in {i + 1} iterations")
+            break
+    if not is_converged:
+        warnings.warn("Harmony did not converge. Consider increasing the number of iterations")
+    if record is not None:
+        record.update(objectives=objectives_harmony, clustering_iters=clustering_iters, R=R, E=E,
+                      O=O, Z_norm=Z_norm, lambda_kb=lam, Y0=Y0, n_clusters=n_clusters,
+                      correction_method=correction_method)
+    return Z_hat
+
+
+def _is_convergent_harmony(objectives_harmony: list, tol: float) -> bool:
+    """_harmony/__init__.py:722-734."""
+    if len(objectives_harmony) < 2:
+        return False
+    obj_old = objectives_harmony[-2]
+    obj_new = objectives_harmony[-1]
+    return (obj_old - obj_new) < tol * np.abs(obj_old)

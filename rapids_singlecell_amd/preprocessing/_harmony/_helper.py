@@ -1,0 +1,61 @@
+"""Host-side helpers of the Harmony driver (batch codes, theta, CSR maps)."""
+
+from __future__ import annotations
+
+import numpy as np
+
+
+def _get_batch_codes(batch_mat, batch_key):
+    """Categorical batch vector.
+
+    Same semantics as preprocessing/_harmony/_helper.py:223-236 of the
+    reference: one key is used as-is; several keys are joined row-wise with
+    "," after ``astype(str)``; the result is ``astype("category")``.
+    """
+    if isinstance(batch_key, str):
+        batch_vec = batch_mat[batch_key]
+    elif len(batch_key) == 1:
+        batch_vec = batch_mat[batch_key[0]]
+    else:
+        df = batch_mat[list(batch_key)].astype("str")
+        batch_vec = df.apply(lambda row: ",".join(row), axis=1)
+    return batch_vec.astype("category")
+
+
+def _create_category_index_mapping(cats: np.ndarray, n_batches: int):
+    """``cat_offsets`` (exclusive cumsum of batch counts) and ``cell_indices``
+    (cells in batch order, ascending index within a batch).
+
+    Same result as _utils/_multi_gpu.py:202-232 of the reference
+    (``lexsort((idx, cats))``), computed on the host: the codes come from
+    pandas on the host anyway and the mapping is integer, bit-exact work.
+    """
+    counts = np.bincount(cats, minlength=n_batches).astype(np.int32)
+    offsets = np.zeros(n_batches + 1, dtype=np.int32)
+    np.cumsum(counts, out=offsets[1:])
+    cell_indices = np.argsort(cats, kind="stable").astype(np.int32)
+    return offsets, cell_indices
+
+
+def _get_theta_array(theta, n_batches: int, dtype) -> np.ndarray:
+    """Theta broadcast to one value per batch.
+
+    preprocessing/_harmony/_helper.py:277-307 of the reference (scalars are
+    broadcast; arrays must have ``n_batches`` elements).
+    """
+    dtype = np.dtype(dtype)
+    if isinstance(theta, (float, int)) and not isinstance(theta, bool):
+        return np.ones(n_batches, dtype=dtype) * float(theta)
+    if isinstance(theta, (list, np.ndarray)):
+        arr = np.asarray(theta, dtype=dtype)
+    elif hasattr(theta, "detach") and hasattr(theta, "cpu"):  # torch tensor
+        arr = theta.detach().cpu().numpy().astype(dtype)
+    else:
+        raise ValueError(
+            f"Theta must be float, int, list, numpy array, or cupy array, got {type(theta)}"
+        )
+    if arr.size != n_batches:
+        raise ValueError(
+            f"Theta array size ({arr.size}) must match number of batches ({n_batches})"
+        )
+    return arr.ravel()
