@@ -1,0 +1,236 @@
+"""Harmony throughput benchmark on one MI355X.
+
+Workload (default): plain FP32 Harmony (harmony1, ridge_lambda=1) on a
+synthetic PCA embedding of N=200,000 cells x 50 PCs from a 30-type Gaussian
+mixture with 8 Dirichlet(1)-sized batches and per-batch shifts ~N(0,1);
+K=100, theta=2, sigma=0.1, block_proportion=0.05, 20 clustering iterations per
+Harmony iteration (early stop disabled).  One step = one Harmony iteration
+(clustering + ridge correction) on device-resident state; the metric is
+cells x Harmony iterations per second.
+
+Also reported: the roofline of the dominant kernel (algorithmic bytes / live
+HIP-event launch time vs 8 TB/s) and a CPU baseline (the NumPy FP64 oracle on
+a bounded sample of the same workload, timed on this host).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "Harmony cells·iters/s (split-BF16 vs FP32, ≤1e-3 rel err) + p50/p99 job latency"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # name: (n_cells, n_pcs, n_types, n_batches, n_clusters)
+    "c2": (200_000, 50, 30, 8, 100),
+    "c3": (1_300_000, 50, 50, 16, 100),
+}
+
+
+def make_data(name: str, seed: int = 0):
+    from rapids_singlecell_amd.datasets import synthetic_pcs
+
+    n, d, types, B, K = WORKLOADS[name]
+    X, obs = synthetic_pcs(n, d, types, B, dirichlet_batches=True, seed=seed,
+                           shift_scale=1.0 if name == "c2" else 0.5)
+    codes = np.asarray(obs["batch"].cat.codes.values, dtype=np.int32)
+    return X, codes, B, K
+
+
+def build_job(X, codes, B, K, precision, stream=None):
+    import torch
+
+    from rapids_singlecell_amd.preprocessing._harmony._job import HarmonyJob
+
+    dev = torch.device("cuda", 0)
+    N_b = np.bincount(codes, minlength=B).astype(np.float32)
+    job = HarmonyJob(torch.as_tensor(X, device=dev), codes, B, n_clusters=K,
+                     theta_np=np.full(B, 2.0, np.float32), N_b_np=N_b, sigma=0.1,
+                     block_proportion=0.05, max_iter_clustering=20,
+                     tol_clustering=float("-inf"), ridge_lambda=1.0, stabilized_penalty=False,
+                     dynamic_lambda=False, correction_method="batched", random_state=0,
+                     precision=precision, stream=stream)
+    return job
+
+
+def time_steps(job, first_iter: int, steps: int) -> float:
+    import torch
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        job.step(first_iter + i)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def roofline(job, first_iter: int, steps: int, esz: int) -> tuple[dict, dict]:
+    """Live HIP-event timing of each kernel kind over `steps` Harmony iterations."""
+    from rapids_singlecell_amd import _native
+
+    _native.profile_collect()
+    _native.profile_enable(True)
+    try:
+        for i in range(steps):
+            job.step(first_iter + i)
+        prof = _native.profile_collect()
+    finally:
+        _native.profile_enable(False)
+    n, d, K = job.n_cells, job.n_pcs, job.n_clusters
+    nblk = -(-n // job.block_size)
+    n_clust = steps * job.max_iter_clustering
+    # algorithmic bytes per launch (reads and writes each array once)
+    bytes_per = {
+        "row": esz * (n / nblk) * (K + d),          # read Z rows, write R rows
+        "centroid": esz * n * (K + d),              # read R and Z
+        "phi": esz * n * (K + d),                   # read R and X
+        "apply": esz * n * (K + 3 * d),             # read R, X; write Z_hat, Z_norm
+    }
+    kernels = {}
+    for kind, (ms, cnt) in prof.items():
+        if cnt == 0:
+            continue
+        entry = {"launches": int(cnt), "total_ms": round(ms, 4), "avg_us": round(1e3 * ms / cnt, 3)}
+        if kind in bytes_per:
+            gbs = bytes_per[kind] / (ms / cnt * 1e-3) / 1e9
+            entry["algorithmic_bytes_per_launch"] = int(bytes_per[kind])
+            entry["achieved_GBs"] = round(gbs, 1)
+        kernels[kind] = entry
+    timed = {k: v for k, v in kernels.items() if "achieved_GBs" in v}
+    dom = max(timed, key=lambda k: timed[k]["total_ms"])
+    ach = timed[dom]["achieved_GBs"]
+    traffic = None
+    tfile = ROOT / "profiles" / "traffic.json"
+    if tfile.exists():
+        try:
+            traffic = json.loads(tfile.read_text()).get(f"{dom}:{n}x{d}x{K}:{esz}")
+        except Exception:
+            traffic = None
+    roof = {"bound": "hbm", "kernel": f"{dom}_kernel", "achieved": ach, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic}
+    kernels["_per_clustering_iter_ms"] = round(
+        sum(v["total_ms"] for k, v in kernels.items()
+            if k in ("row", "centroid", "yreduce", "order")) / max(n_clust, 1), 4)
+    return roof, kernels
+
+
+def cpu_baseline(X, codes, B, K, n_sample: int) -> dict:
+    """NumPy FP64 oracle: 1 Harmony iteration (20 clustering iterations) on the
+    first `n_sample` cells of the same workload."""
+    from oracle import harmony_oracle as ho
+
+    try:
+        from threadpoolctl import threadpool_info
+
+        blas_threads = max((i.get("num_threads", 1) for i in threadpool_info()), default=1)
+    except Exception:  # noqa: BLE001
+        blas_threads = None
+    Xs = X[:n_sample].astype(np.float64)
+    cs = codes[:n_sample]
+    Z = ho.l2_normalize(Xs)
+    Y0 = Z[np.random.default_rng(0).choice(n_sample, K, replace=False)]
+    t0 = time.perf_counter()
+    ho.harmonize(Xs, cs, B, Y0=Y0, n_clusters=K, max_iter_harmony=1, max_iter_clustering=20,
+                 tol_harmony=float("-inf"), tol_clustering=float("-inf"),
+                 stabilized_penalty=False, dynamic_lambda=False, ridge_lambda=1.0,
+                 random_state=0)
+    dt = time.perf_counter() - t0
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except Exception:  # noqa: BLE001
+        aff = os.cpu_count()
+    return {"value": round(n_sample / dt, 1), "unit": "cells·iters/s",
+            "cores": blas_threads or aff, "kind": "port",
+            "sample": f"oracle FP64 harmonize, first {n_sample} cells of the workload, "
+                      f"1 Harmony iter x 20 clustering iters, {dt:.1f} s "
+                      f"(os.cpu_count={os.cpu_count()}, affinity={aff}, blas_threads={blas_threads})"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default="native", choices=["native", "split_bf16"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=50_000)
+    ap.add_argument("--no-roofline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the Harmony path "
+              "runs on a single MI355X", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from rapids_singlecell_amd import _native
+
+    _native.load()
+    prec = {"native": 0, "split_bf16": 1}[args.precision]
+    X, codes, B, K = make_data(args.workload)
+    n, d = X.shape
+    print(f"[bench] workload {args.workload}: N={n} d={d} K={K} B={B} precision={args.precision}",
+          file=sys.stderr, flush=True)
+    job = build_job(X, codes, B, K, prec)
+    job.init()
+    torch.cuda.synchronize()
+    it = 0
+    for _ in range(args.warmup):
+        job.step(it)
+        it += 1
+    torch.cuda.synchronize()
+    print("[bench] warmup done", file=sys.stderr, flush=True)
+    dt = time_steps(job, it, args.steps)
+    it += args.steps
+    value = n * args.steps / dt
+    print(f"[bench] timed {args.steps} steps in {dt:.3f} s", file=sys.stderr, flush=True)
+    roof, kernels = (None, None)
+    if not args.no_roofline:
+        roof, kernels = roofline(job, it, 2, X.dtype.itemsize)
+        it += 2
+    cpu = None
+    if not args.no_cpu:
+        print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(X, codes, B, K, args.cpu_sample)
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "cells·iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * dt / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: Gaussian-mixture PCA embedding with Dirichlet batch sizes and batch shifts",
+        "config": {
+            "workload": f"harmony1 N={n} d={d} K={K} B={B} theta=2 sigma=0.1 lambda=1 "
+                        f"block_proportion=0.05, 20 clustering iters per Harmony iter",
+            "precision": args.precision,
+            "n_cells": n, "n_pcs": d, "n_clusters": K, "n_batches": B,
+        },
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "kernels": kernels,
+    }
+    if cpu:
+        out["speedup_vs_cpu"] = round(value / cpu["value"], 1)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -59,6 +59,22 @@ typedef struct hz_dims {
 int hz_version(void);
 const char* hz_last_error(void);
 
+/* Opt-in HIP-event timing of the library's own launches (benchmark
+ * instrumentation; no counterpart in the reference).  While enabled, each
+ * timed launch is bracketed by events on its stream; hz_profile_collect waits
+ * for them, writes the summed milliseconds and launch counts per kind
+ * (HZ_PROF_*) and resets the counters. */
+#define HZ_PROF_ROW 0        /* row kernel, one launch per block */
+#define HZ_PROF_CENTROID 1   /* centroid pass */
+#define HZ_PROF_YREDUCE 2    /* centroid reduction + block-0 setup */
+#define HZ_PROF_ORDER 3      /* block order pipeline (several launches) */
+#define HZ_PROF_CORRECTION 4 /* whole correction */
+#define HZ_PROF_PHI 5        /* correction: Phi pass */
+#define HZ_PROF_APPLY 6      /* correction: apply + renormalise */
+#define HZ_PROF_KINDS 8
+int hz_profile_enable(int on);
+int hz_profile_collect(double* ms_out, long long* launches_out, int n_kinds);
+
 /* Workspace for hz_init_clusters / hz_clustering_loop / hz_correction.
  * Replaces _allocate_clustering_workspace (_harmony/__init__.py:356-384),
  * get_cub_sort_temp_bytes (_cuda/harmony/clustering/clustering.cu:48-59) and

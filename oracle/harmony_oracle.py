@@ -25,6 +25,13 @@ SUPPRESS_PENALTY = 1e30
 _U32 = 0xFFFFFFFF
 
 
+def _scatter_rows(cats: np.ndarray, rows: np.ndarray, n_groups: int) -> np.ndarray:
+    """sum of ``rows`` per group id (``np.add.at`` semantics, as one product)."""
+    onehot = np.zeros((n_groups, len(cats)), dtype=rows.dtype)
+    onehot[cats, np.arange(len(cats))] = 1
+    return onehot @ rows
+
+
 # --------------------------------------------------------------------------
 # Row normalisation
 # --------------------------------------------------------------------------
@@ -220,8 +227,7 @@ def init_clusters(Z_norm, Y0, *, sigma, Pr_b, theta, cats, n_batches,
     R = np.exp(term * (dt.type(1) - sim))
     R = l1_normalize(R)
     E = Pr_b.reshape(-1, 1) * R.sum(axis=0)[None, :]
-    O = np.zeros((n_batches, R.shape[1]), dtype=dt)
-    np.add.at(O, cats, R)
+    O = _scatter_rows(cats, R, n_batches)
     obj = objective(R, sim, O, E, theta, sigma, stabilized)
     return R, E.astype(dt), O, float(obj)
 
@@ -263,14 +269,14 @@ def clustering_loop(Z_norm, R, E, O, Pr_b, cats, theta, *, sigma, tol,
             R_in = R[idx]
             c_in = cats[idx]
             R_in_sum = R_in.sum(axis=0)
-            np.subtract.at(O, c_in, R_in)
+            O -= _scatter_rows(c_in, R_in, O.shape[0])
             E -= Pr[:, None] * R_in_sum[None, :]
             pen = penalty(E, O, theta, stabilized)
             v = np.exp(term * (dt(1) - sim[idx])) * pen[c_in]
             R_out = v * (dt(1) / v.sum(axis=1))[:, None]
             R[idx] = R_out
             R_out_sum = R_out.sum(axis=0)
-            np.add.at(O, c_in, R_out)
+            O += _scatter_rows(c_in, R_out, O.shape[0])
             E += Pr[:, None] * R_out_sum[None, :]
         obj = objective(R, sim, O, E, theta, sigma, stabilized)
         objectives.append(obj)
@@ -421,8 +427,7 @@ def kmeans_centroids(Z_norm, n_clusters: int, random_state: int, max_iter: int =
     for _ in range(max_iter):
         score = Z_norm @ Y.T - dt.type(0.5) * np.einsum("kd,kd->k", Y, Y)[None, :]
         lab = np.argmax(score, axis=1)
-        sums = np.zeros_like(Y)
-        np.add.at(sums, lab, Z_norm)
+        sums = _scatter_rows(lab, Z_norm, n_clusters)
         cnt = np.bincount(lab, minlength=n_clusters)
         nz = cnt > 0
         Y[nz] = sums[nz] / cnt[nz, None].astype(dt)

@@ -19,6 +19,7 @@ HZ_F64 = 8
 HZ_PREC_NATIVE = 0
 HZ_PREC_SPLIT_BF16 = 1
 HZ_CORR = {"batched": 0, "fast": 1, "original": 2}
+PROF_KINDS = ["row", "centroid", "yreduce", "order", "correction", "phi", "apply", "unused"]
 
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
@@ -45,6 +46,8 @@ _P = ctypes.POINTER(HzDims)
 # name -> (restype, argtypes)
 _SIGNATURES = {
     "hz_version": (_i, []),
+    "hz_profile_enable": (_i, [_i]),
+    "hz_profile_collect": (_i, [ctypes.POINTER(_d), ctypes.POINTER(_ll), _i]),
     "hz_last_error": (ctypes.c_char_p, []),
     "hz_workspace_bytes": (_sz, [_P]),
     "hz_l2_normalize": (_i, [_i, _vp, _vp, _ll, _i, _vp]),
@@ -121,3 +124,16 @@ def check(rc: int, what: str) -> None:
 def dims(n_cells, n_pcs, n_clusters, n_batches, block_size, dtype_size, precision=0):
     return HzDims(int(n_cells), int(n_pcs), int(n_clusters), int(n_batches),
                   int(block_size), int(dtype_size), int(precision))
+
+
+def profile_enable(on: bool) -> None:
+    load().hz_profile_enable(int(bool(on)))
+
+
+def profile_collect() -> dict:
+    """Summed launch milliseconds and launch counts per kind since the last call."""
+    lib = load()
+    ms = (_d * len(PROF_KINDS))()
+    cnt = (_ll * len(PROF_KINDS))()
+    check(lib.hz_profile_collect(ms, cnt, len(PROF_KINDS)), "hz_profile_collect")
+    return {k: (ms[i], cnt[i]) for i, k in enumerate(PROF_KINDS)}

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <vector>
 
 #include "hz_correct.hpp"
 
@@ -54,6 +55,70 @@ int device_cu_count() {
   return g_cus[dev];
 }
 
+// ---------------------------------------------------------------- launch timing
+//
+// Opt-in HIP-event timing of the library's own launches (benchmarks only).
+// Events bracket each timed launch on the stream it runs on; hz_profile_collect
+// synchronises them and sums the elapsed times per kind.
+
+constexpr int kProfKinds = 8;  // HZ_PROF_* in harmony_hip.h
+struct ProfRec {
+  hipEvent_t a, b;
+  int kind;
+};
+static bool g_prof = false;
+static std::vector<ProfRec> g_prof_recs;
+static std::vector<hipEvent_t> g_prof_pool;
+static double g_prof_ms[kProfKinds];
+static long long g_prof_n[kProfKinds];
+
+static hipEvent_t prof_event() {
+  if (!g_prof_pool.empty()) {
+    hipEvent_t e = g_prof_pool.back();
+    g_prof_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Scope object: records a start event now and an end event when destroyed.
+struct ProfScope {
+  hipStream_t s;
+  int kind;
+  hipEvent_t a = nullptr;
+  ProfScope(int k, hipStream_t st) : s(st), kind(k) {
+    if (g_prof) {
+      a = prof_event();
+      if (a) hipEventRecord(a, s);
+    }
+  }
+  ~ProfScope() {
+    if (g_prof && a) {
+      hipEvent_t b = prof_event();
+      if (b) {
+        hipEventRecord(b, s);
+        g_prof_recs.push_back({a, b, kind});
+      }
+    }
+  }
+};
+
+static int prof_collect() {
+  for (auto& r : g_prof_recs) {
+    HZ_HIP(hipEventSynchronize(r.b));
+    float ms = 0.f;
+    HZ_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+    g_prof_ms[r.kind] += ms;
+    g_prof_n[r.kind] += 1;
+    g_prof_pool.push_back(r.a);
+    g_prof_pool.push_back(r.b);
+  }
+  g_prof_recs.clear();
+  return HZ_OK;
+}
+
 static int pick_ct(int K) {
   if (K <= 64) return 4;
   if (K <= 128) return 8;
@@ -98,8 +163,6 @@ struct Layout {
   void* Y;
   void* YT;
   void* pen;
-  double* scratch;
-  double* cs;
   // correction
   void* phipart;
   double* Phi;
@@ -127,10 +190,14 @@ static int make_layout(const hz_dims* dm, void* base, Layout& L) {
   L.bs = std::min(dm->block_size, dm->n_cells);
   L.esz = dm->dtype;
   L.CT = pick_ct(L.K);
-  L.DT = pick_dt(L.d);
+  L.DT = pick_dt(L.d + 1);  // one spare column: the centroid pass sums R through it
   HZ_RET_IF(!L.CT, HZ_ERR_UNSUPPORTED, "n_clusters=%d > 256 is not supported", L.K);
-  HZ_RET_IF(!L.DT, HZ_ERR_UNSUPPORTED, "n_pcs=%d > 128 is not supported", L.d);
-  HZ_RET_IF(L.B > 16384, HZ_ERR_UNSUPPORTED, "n_batches=%d > 16384 is not supported", L.B);
+  HZ_RET_IF(!L.DT, HZ_ERR_UNSUPPORTED, "n_pcs=%d > 127 is not supported", L.d);
+  {
+    const int need = (L.esz == 4) ? block_update_min_lds<float>(L.B) : block_update_min_lds<double>(L.B);
+    HZ_RET_IF(need > kYRedLds, HZ_ERR_UNSUPPORTED, "n_batches=%d is too many (limit %d)", L.B,
+              (kYRedLds - 16) / (16 + 2 * L.esz));
+  }
   L.Kp = 16 * L.CT;
   L.Dp = 16 * L.DT;
   L.chunk = (L.bs <= 32768) ? 64 : 128;
@@ -180,8 +247,6 @@ static int make_layout(const hz_dims* dm, void* base, Layout& L) {
   L.Y = c.take<char>((size_t)KD * L.esz);
   L.YT = c.take<char>((size_t)L.d * L.Kp * L.esz);
   L.pen = c.take<char>((size_t)BK * L.esz);
-  L.scratch = c.take<double>(BK);
-  L.cs = c.take<double>(L.K);
   L.phipart = c.take<char>((size_t)L.spd.maxch * KD * L.esz);
   L.Phi = c.take<double>(BK * L.d);
   L.rsum = c.take<long long>(BK);
@@ -204,8 +269,6 @@ static EpiArgs<T> make_epi(const Layout& L, T* O, T* E, const T* Pr, const T* th
   e.pen = reinterpret_cast<T*>(L.pen);
   e.acc_in = L.acc_in;
   e.acc_out = L.acc_out;
-  e.scratch = L.scratch;
-  e.cs = L.cs;
   e.ctrl = L.ctrl;
   e.last_obj = last_obj;
   e.B = L.B;
@@ -230,8 +293,9 @@ static int check_ws(const Layout& L, void* ws, size_t ws_bytes) {
 
 template <typename T>
 static size_t row_lds_bytes(const Layout& L) {
-  return (size_t)L.d * L.Kp * sizeof(T) + (size_t)L.d * kZtStride * sizeof(T) +
-         (size_t)(kBlock / kWave) * L.Kp * sizeof(double);
+  size_t need = (size_t)L.d * L.Kp * sizeof(T) + (size_t)L.d * kZtStride * sizeof(T) +
+                (size_t)(kBlock / kWave) * L.Kp * sizeof(double);
+  return std::max(need, (size_t)block_update_min_lds<T>(L.B));
 }
 
 template <typename T, int CT>
@@ -246,8 +310,9 @@ static int launch_row_ct(const RowArgs<T>& a, int grid, size_t lds, hipStream_t 
 }
 
 template <typename T>
-static int launch_row(const Layout& L, const RowArgs<T>& a, int grid, hipStream_t s) {
+static int launch_row(const Layout& L, RowArgs<T> a, int grid, hipStream_t s) {
   const size_t lds = row_lds_bytes<T>(L);
+  a.smem_bytes = (int)lds;
   HZ_RET_IF(lds > (size_t)device_lds_limit(), HZ_ERR_UNSUPPORTED,
             "row kernel needs %zu bytes of LDS (limit %d): reduce n_pcs or n_clusters", lds,
             device_lds_limit());
@@ -388,9 +453,8 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
   ya.K = L.K;
   ya.d = L.d;
   ya.Kp = L.Kp;
-  ya.counter = L.counters + L.pd.nblk;
   ya.epi = epi;
-  const int ygrid = (int)std::max<long long>(1, std::min<long long>(ceil_div((long long)L.K * L.d, kBlock), 64));
+  const int ygrid = L.K;
 
   RowArgs<T> ra;
   ra.Z = Z;
@@ -421,17 +485,27 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
       HZ_TRY(read_done(L.ctrl, s, &done));
       if (done) break;
     }
-    HZ_TRY(launch_block_order(L.pd, seed + (uint32_t)it, cats, L.pb, L.ct, L.nch_tmp, &L.ctrl->done,
-                              s));
-    HZ_TRY(launch_centroid<T>(L, ca, L.G, s));
-    hipLaunchKernelGGL(yreduce_kernel<T>, dim3(ygrid), dim3(kBlock), 0, s, ya);
-    HZ_LAUNCHED("yreduce_kernel");
+    {
+      ProfScope ps(HZ_PROF_ORDER, s);
+      HZ_TRY(launch_block_order(L.pd, seed + (uint32_t)it, cats, L.pb, L.ct, L.nch_tmp,
+                                &L.ctrl->done, s));
+    }
+    {
+      ProfScope ps(HZ_PROF_CENTROID, s);
+      HZ_TRY(launch_centroid<T>(L, ca, L.G, s));
+    }
+    {
+      ProfScope ps(HZ_PROF_YREDUCE, s);
+      hipLaunchKernelGGL(yreduce_kernel<T>, dim3(ygrid), dim3(kBlock), 0, s, ya);
+      HZ_LAUNCHED("yreduce_kernel");
+    }
     for (int j = 0; j < L.pd.nblk; ++j) {
       const int rows_j = std::min(L.bs, L.n - j * L.bs);
       const int grid = (int)(ceil_div(rows_j, L.chunk) + L.B);
       ra.seg_base = j * L.B;
       ra.block = j;
       ra.counter = L.counters + j;
+      ProfScope ps(HZ_PROF_ROW, s);
       HZ_TRY(launch_row<T>(L, ra, grid, s));
     }
   }
@@ -465,6 +539,7 @@ static int correction_impl(const hz_dims* dm, const T* X, const T* R, const T* O
   HZ_TRY(make_layout(dm, ws, L));
   HZ_RET_IF(method < HZ_CORR_BATCHED || method > HZ_CORR_ORIGINAL, HZ_ERR_ARG,
             "unknown correction method %d", method);
+  ProfScope ps_all(HZ_PROF_CORRECTION, s);
   T* lam = lam_out ? lam_out : reinterpret_cast<T*>(L.lam);
   const int BK = L.B * L.K;
   hipLaunchKernelGGL(lambda_kernel<T>, dim3((int)ceil_div(BK, 256)), dim3(256), 0, s, E, O, Nb,
@@ -494,7 +569,10 @@ static int correction_impl(const hz_dims* dm, const T* X, const T* R, const T* O
   ca.d = L.d;
   ca.K = L.K;
   ca.scale_o = L.scale_o;
-  HZ_TRY(launch_centroid<T>(L, ca, (int)L.spd.maxch, s));
+  {
+    ProfScope ps(HZ_PROF_PHI, s);
+    HZ_TRY(launch_centroid<T>(L, ca, (int)L.spd.maxch, s));
+  }
   {
     long long tot = (long long)BK * L.d;
     int g = (int)std::max<long long>(1, std::min<long long>(ceil_div(tot, 256), 4096));
@@ -544,6 +622,7 @@ static int correction_impl(const hz_dims* dm, const T* X, const T* R, const T* O
   HZ_RET_IF(lds > (size_t)device_lds_limit(), HZ_ERR_UNSUPPORTED,
             "apply kernel needs %zu bytes of LDS", lds);
   const int grid = (int)L.sd.maxch;
+  ProfScope ps(HZ_PROF_APPLY, s);
   switch (L.DT) {
     case 2: return launch_apply_dt<T, 2>(L, aa, grid, lds, s);
     case 4: return launch_apply_dt<T, 4>(L, aa, grid, lds, s);
@@ -732,6 +811,24 @@ using namespace hz;
 extern "C" {
 
 int hz_version(void) { return 100; }
+
+int hz_profile_enable(int on) {
+  g_prof = on != 0;
+  return HZ_OK;
+}
+
+int hz_profile_collect(double* ms_out, long long* launches_out, int n_kinds) {
+  HZ_TRY(prof_collect());
+  for (int k = 0; k < kProfKinds; ++k) {
+    if (k < n_kinds) {
+      if (ms_out) ms_out[k] = g_prof_ms[k];
+      if (launches_out) launches_out[k] = g_prof_n[k];
+    }
+    g_prof_ms[k] = 0.0;
+    g_prof_n[k] = 0;
+  }
+  return HZ_OK;
+}
 
 const char* hz_last_error(void) { return hz::g_err; }
 

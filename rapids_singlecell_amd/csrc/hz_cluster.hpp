@@ -16,7 +16,7 @@
 //    row kernel from the rows it writes (each row is written exactly once per
 //    iteration, with the start-of-iteration similarities).
 //  * O/E updates, the penalty of the next block and the convergence test run
-//    in the last workgroup to finish each row-kernel launch.
+//    in the last workgroup to finish each row-kernel launch, staged in LDS.
 #pragma once
 
 #include "hz_perm.hpp"
@@ -40,20 +40,16 @@ struct EpiArgs {
   T* E;
   const T* Pr;
   const T* theta;
-  T* pen;            // B x K penalty for the next block
+  T* pen;              // B x K penalty for the next block
   long long* acc_in;   // nseg x K
   long long* acc_out;  // nseg x K
-  double* scratch;   // B x K
-  double* cs;        // K
   Ctrl* ctrl;
-  T* last_obj;       // device scalar written after every iteration
+  T* last_obj;         // device scalar written after every iteration
   int B, K, nblk;
   T sigma, tol;
   int stabilized;
   double inv_scale_o, inv_scale_obj;
 };
-
-// ---------------------------------------------------------------- epilogue
 
 template <typename T>
 __device__ __forceinline__ T penalty_value(T E, T O, T th, int stabilized) {
@@ -61,99 +57,126 @@ __device__ __forceinline__ T penalty_value(T E, T O, T th, int stabilized) {
   return dpow((E + T(1)) / denom, th);
 }
 
-// Move a block's fixed-point O contribution into O and E.
-// mode 0: O += d, E += Pr (x) colsum(d)   (block written)
-// mode 1: O -= d, E -= Pr (x) colsum(d)   (block about to be rewritten)
-// mode 2: O  = d, E  = Pr (x) colsum(d)   (initialisation)
-template <typename T>
-__device__ void epi_apply(const EpiArgs<T>& a, long long* acc, int seg0, int mode) {
-  const int BK = a.B * a.K;
-  for (int p = threadIdx.x; p < BK; p += blockDim.x) {
-    const int b = p / a.K, k = p - b * a.K;
-    long long* q = acc + (long long)(seg0 + b) * a.K + k;
-    long long v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(q, 0LL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    double d = from_fixed(v, a.inv_scale_o);
-    a.scratch[p] = d;
-    T dt = T(d);
-    if (mode == 0)
-      a.O[p] += dt;
-    else if (mode == 1)
-      a.O[p] -= dt;
-    else
-      a.O[p] = dt;
-  }
-  __syncthreads();
-  for (int k = threadIdx.x; k < a.K; k += blockDim.x) {
-    double s = 0.0;
-    for (int b = 0; b < a.B; ++b) s += a.scratch[b * a.K + k];
-    a.cs[k] = s;
-  }
-  __syncthreads();
-  for (int p = threadIdx.x; p < BK; p += blockDim.x) {
-    const int b = p / a.K, k = p - b * a.K;
-    T upd = a.Pr[b] * T(a.cs[k]);
-    if (mode == 0)
-      a.E[p] += upd;
-    else if (mode == 1)
-      a.E[p] -= upd;
-    else
-      a.E[p] = upd;
-  }
-  __syncthreads();
-}
+// ---------------------------------------------------------------- epilogue
 
-template <typename T>
-__device__ void epi_penalty(const EpiArgs<T>& a) {
-  const int BK = a.B * a.K;
-  for (int p = threadIdx.x; p < BK; p += blockDim.x) {
-    const int b = p / a.K;
-    a.pen[p] = penalty_value(a.E[p], a.O[p], a.theta[b], a.stabilized);
-  }
-}
+enum : int {
+  kUpdOut = 1,        // add the written block's contribution (acc_out)
+  kUpdIn = 2,         // remove the next block's old contribution (acc_in)
+  kUpdOverwrite = 4,  // O = contribution, E = Pr (x) colsum (initialisation)
+  kUpdPen = 8,        // penalty for the next block
+  kUpdObj = 16,       // objective of the finished iteration (+ convergence test)
+  kUpdObjInit = 32,   // objective of the initialisation
+};
 
-// sigma * sum theta_b O log(numer / (E + 1)), kernels_clustering.cuh:69-103.
+// O/E bookkeeping between blocks, in the order of clustering.cu:321-371:
+//   O += scatter(R_out_j); E += Pr (x) colsum(R_out_j);
+//   O -= scatter(R_in_{j+1}); E -= Pr (x) colsum(R_in_{j+1}); penalty.
+// Block contributions arrive as fixed-point sums (acc_out / acc_in); columns
+// are processed in slabs that fit `smem_bytes` of LDS.  Every thread of the
+// workgroup must call it.  Returns the objective when kUpdObj/kUpdObjInit.
 template <typename T>
-__device__ double epi_diversity(const EpiArgs<T>& a, double* red) {
-  const int BK = a.B * a.K;
-  double acc = 0.0;
-  for (int p = threadIdx.x; p < BK; p += blockDim.x) {
-    const int b = p / a.K;
-    T O = a.O[p], E = a.E[p];
-    T numer = a.stabilized ? (O + E + T(1)) : (O + T(1));
-    T ratio = numer / (E + T(1));
-    acc += (double)(a.theta[b] * O * dlog(ratio));
+__device__ T block_update(const EpiArgs<T>& a, unsigned char* smem, int smem_bytes, int seg_out,
+                          int seg_in, int flags, double* red) {
+  const int B = a.B, K = a.K;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int per_col = B * (16 + 2 * (int)sizeof(T)) + 16;
+  const int KS = max(1, min(K, smem_bytes / per_col));
+  double* s_do = reinterpret_cast<double*>(smem);
+  double* s_di = s_do + (size_t)B * KS;
+  double* s_co = s_di + (size_t)B * KS;
+  double* s_ci = s_co + KS;
+  T* s_O = reinterpret_cast<T*>(s_ci + KS);
+  T* s_E = s_O + (size_t)B * KS;
+  const double inv = a.inv_scale_o;
+  double div = 0.0;
+  for (int k0 = 0; k0 < K; k0 += KS) {
+    const int ks = min(KS, K - k0);
+    const int n = B * ks;
+    __syncthreads();
+    for (int q = tid; q < n; q += nt) {
+      const int b = q / ks, kk = q - b * ks;
+      const long long p = (long long)b * K + k0 + kk;
+      double vo = 0.0, vi = 0.0;
+      if (flags & (kUpdOut | kUpdOverwrite)) {
+        long long* src = a.acc_out + (long long)(seg_out + b) * K + k0 + kk;
+        vo = (double)load_agent(src) * inv;
+        store_agent(src, 0);
+      }
+      if (flags & kUpdIn) {
+        long long* src = a.acc_in + (long long)(seg_in + b) * K + k0 + kk;
+        vi = (double)load_agent(src) * inv;
+        store_agent(src, 0);
+      }
+      s_do[q] = vo;
+      s_di[q] = vi;
+      if (!(flags & kUpdOverwrite)) {
+        s_O[q] = a.O[p];
+        s_E[q] = a.E[p];
+      }
+    }
+    __syncthreads();
+    for (int kk = tid; kk < ks; kk += nt) {
+      double co = 0.0, ci = 0.0;
+      for (int b = 0; b < B; ++b) {
+        co += s_do[b * ks + kk];
+        ci += s_di[b * ks + kk];
+      }
+      s_co[kk] = co;
+      s_ci[kk] = ci;
+    }
+    __syncthreads();
+    for (int q = tid; q < n; q += nt) {
+      const int b = q / ks, kk = q - b * ks;
+      const long long p = (long long)b * K + k0 + kk;
+      const T pr = a.Pr[b];
+      T o, e;
+      if (flags & kUpdOverwrite) {
+        o = T(s_do[q]);
+        e = pr * T(s_co[kk]);
+      } else {
+        o = s_O[q];
+        e = s_E[q];
+        if (flags & kUpdOut) {
+          o = o + T(s_do[q]);
+          e = e + pr * T(s_co[kk]);
+        }
+        if (flags & kUpdIn) {
+          o = o - T(s_di[q]);
+          e = e - pr * T(s_ci[kk]);
+        }
+      }
+      a.O[p] = o;
+      a.E[p] = e;
+      if (flags & kUpdPen) a.pen[p] = penalty_value(e, o, a.theta[b], a.stabilized);
+      if (flags & (kUpdObj | kUpdObjInit)) {
+        // sigma * theta_b * O * log(numer / (E + 1)), kernels_clustering.cuh:69-103
+        T numer = a.stabilized ? (o + e + T(1)) : (o + T(1));
+        div += (double)(a.theta[b] * o * dlog(numer / (e + T(1))));
+      }
+    }
   }
-  return block_sum(acc, red);
-}
-
-// Objective of the finished iteration, history and window-3 convergence test
-// (clustering.cu:375-387).  Returns the objective.
-template <typename T>
-__device__ T epi_objective(const EpiArgs<T>& a, double* red, bool record) {
-  double div = epi_diversity(a, red);
+  if (!(flags & (kUpdObj | kUpdObjInit))) return T(0);
+  div = block_sum(div, red);
   __shared__ T s_obj;
-  if (threadIdx.x == 0) {
-    long long* oa = a.ctrl->obj_acc;
-    double km = from_fixed(__hip_atomic_load(&oa[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
-                           a.inv_scale_obj);
-    double ent = from_fixed(__hip_atomic_load(&oa[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
-                            a.inv_scale_obj);
-    __hip_atomic_store(&oa[0], 0LL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(&oa[1], 0LL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (tid == 0) {
+    Ctrl* c = a.ctrl;
+    double km = (double)load_agent(&c->obj_acc[0]) * a.inv_scale_obj;
+    double ent = (double)load_agent(&c->obj_acc[1]) * a.inv_scale_obj;
+    store_agent(&c->obj_acc[0], 0);
+    store_agent(&c->obj_acc[1], 0);
     T obj = T(km + (double)a.sigma * ent + (double)a.sigma * div);
     s_obj = obj;
-    if (record) {
-      Ctrl* c = a.ctrl;
-      int n = c->n_obj + 1;
-      c->n_obj = n;
-      c->hist[(n - 1) & 3] = (double)obj;
+    if (flags & kUpdObj) {
+      // history + window-3 test, clustering.cu:375-387
+      int nobj = c->n_obj + 1;
+      c->n_obj = nobj;
+      c->hist[(nobj - 1) & 3] = (double)obj;
       if (a.last_obj) *a.last_obj = obj;
-      if (n >= 4) {
+      if (nobj >= 4) {
         T old_s = T(0), new_s = T(0);
         for (int w = 0; w < 3; ++w) {
-          old_s = old_s + T(c->hist[(n - 2 - w) & 3]);
-          new_s = new_s + T(c->hist[(n - 1 - w) & 3]);
+          old_s = old_s + T(c->hist[(nobj - 2 - w) & 3]);
+          new_s = new_s + T(c->hist[(nobj - 1 - w) & 3]);
         }
         if ((old_s - new_s) < a.tol * fabs(old_s)) c->done = 1;
       }
@@ -163,20 +186,27 @@ __device__ T epi_objective(const EpiArgs<T>& a, double* red, bool record) {
   return s_obj;
 }
 
+template <typename T>
+__host__ __device__ inline int block_update_min_lds(int B) {
+  return B * (16 + 2 * (int)sizeof(T)) + 16;
+}
+
 // ---------------------------------------------------------------- arrival
 
 // Returns true in every thread of the last workgroup of the grid to arrive.
+// The only data the last workgroup reads from the others are fixed-point
+// accumulators updated with fixed_add_wait (device-scope atomics whose results
+// were waited for before this barrier) and read back with load_agent, so a
+// relaxed device-scope counter suffices; no device-wide release fence (an L2
+// write-back per workgroup) is needed.
 __device__ __forceinline__ bool last_to_arrive(int* counter, int* s_flag) {
-  __threadfence();
   __syncthreads();
   if (threadIdx.x == 0) {
-    int prev = atomicAdd(counter, 1);
+    int prev = __hip_atomic_fetch_add(counter, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     *s_flag = (prev == (int)gridDim.x - 1) ? 1 : 0;
   }
   __syncthreads();
-  bool last = *s_flag != 0;
-  if (last) __threadfence();
-  return last;
+  return *s_flag != 0;
 }
 
 __device__ __forceinline__ void reset_counter(int* counter) {
@@ -185,16 +215,13 @@ __device__ __forceinline__ void reset_counter(int* counter) {
 
 // ---------------------------------------------------------------- centroids
 
-// Normalised centroids, transposed and zero padded: YT[c * Kp + k].
+// Normalised centroids, transposed and zero padded: YT[c * Kp + k]
+// (kernels_normalize.cuh:6-54 semantics).  `s_inv` needs Kp entries of LDS.
 template <typename T>
-__device__ void normalize_centroids(const T* Y, T* YT, int K, int d, int Kp) {
+__device__ void normalize_centroids(const T* Y, T* YT, int K, int d, int Kp, T* s_inv) {
   const int lane = threadIdx.x & (kWave - 1);
   const int w = threadIdx.x / kWave, nw = blockDim.x / kWave;
-  for (int k = w; k < Kp; k += nw) {
-    if (k >= K) {
-      for (int c = lane; c < d; c += kWave) YT[(long long)c * Kp + k] = T(0);
-      continue;
-    }
+  for (int k = w; k < K; k += nw) {
     T ss = T(0);
     for (int c = lane; c < d; c += kWave) {
       T v = Y[(long long)k * d + c];
@@ -203,7 +230,12 @@ __device__ void normalize_centroids(const T* Y, T* YT, int K, int d, int Kp) {
     ss = wave_sum(ss);
     T inv = T(1) / sqrt(ss);
     if (!(inv <= T(1e12))) inv = T(1e12);
-    for (int c = lane; c < d; c += kWave) YT[(long long)c * Kp + k] = Y[(long long)k * d + c] * inv;
+    if (lane == 0) s_inv[k] = inv;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < d * Kp; e += blockDim.x) {
+    const int c = e / Kp, k = e - c * Kp;
+    YT[e] = (k < K) ? Y[(long long)k * d + c] * s_inv[k] : T(0);
   }
 }
 
@@ -223,66 +255,107 @@ struct CentArgs {
   double scale_o;
 };
 
-// Y_partial = R^T Z over a contiguous range of chunks, and each chunk's
-// column sums of R into its (block, batch) O-contribution.
+// Y_partial = R^T Z over a contiguous range of chunks (64-row tiles staged in
+// LDS, the next tile prefetched into registers), plus each (block, batch)
+// segment's column sums of R into acc_in.  The column sums come out of the
+// same accumulation: column d of the Z tile holds 1 for every valid row.
 template <typename T, int KT, int DT>
 __global__ __launch_bounds__(kBlock) void centroid_kernel(CentArgs<T> a) {
   constexpr int Kp = 16 * KT, Dp = 16 * DT;
-  // rows per LDS tile: keep the two tiles within 64 KB
   constexpr int TR = (64 * (Kp + Dp) * (int)sizeof(T) <= 65536)   ? 64
                      : (32 * (Kp + Dp) * (int)sizeof(T) <= 65536) ? 32
                                                                   : 16;
+  constexpr int MR = TR * Kp / kBlock;  // R elements per thread per tile
+  constexpr int MZ = TR * Dp / kBlock;  // Z elements per thread per tile
   __shared__ __align__(16) T Rs[TR * Kp];
   __shared__ __align__(16) T Zs[TR * Dp];
-  __shared__ int idx_s[TR];
   if (a.ctrl->done) return;
   const int tid = threadIdx.x, kg = tid >> 4, dg = tid & 15;
   const int d = a.d, K = a.K;
-  for (int e = tid; e < TR * Kp; e += kBlock) Rs[e] = T(0);
-  for (int e = tid; e < TR * Dp; e += kBlock) Zs[e] = T(0);
   const int nch = a.seg_ch[a.nseg];
   const int c0 = blockIdx.x * a.cpw;
   const int c1 = min(nch, c0 + a.cpw);
+
   T acc[KT][DT];
 #pragma unroll
   for (int i = 0; i < KT; ++i)
 #pragma unroll
     for (int j = 0; j < DT; ++j) acc[i][j] = T(0);
-  for (int c = c0; c < c1; ++c) {
-    const int seg = a.ch_seg[c], p0 = a.ch_start[c], len = a.ch_len[c];
-    double csum = 0.0;
-    for (int r0 = 0; r0 < len; r0 += TR) {
-      const int rows = min(TR, len - r0);
-      __syncthreads();
-      if (tid < TR) idx_s[tid] = (tid < rows) ? a.order[p0 + r0 + tid] : -1;
-      __syncthreads();
-      for (int e = tid; e < TR * K; e += kBlock) {
-        const int r = e / K, k = e - r * K;
-        const int i = idx_s[r];
-        Rs[r * Kp + k] = (i >= 0) ? a.R[(long long)i * K + k] : T(0);
+
+  T rv[MR], zv[MZ];
+  // tile cursor: chunk c, row offset r0 inside the chunk
+  auto fetch = [&](int c, int r0) {
+    const int p0 = a.ch_start[c] + r0;
+    const int rows = min(TR, a.ch_len[c] - r0);
+#pragma unroll
+    for (int m = 0; m < MR; ++m) {
+      const int e = tid + m * kBlock;
+      const int r = e / Kp, k = e % Kp;
+      T v = T(0);
+      if (r < rows && k < K) v = a.R[(long long)a.order[p0 + r] * K + k];
+      rv[m] = v;
+    }
+#pragma unroll
+    for (int m = 0; m < MZ; ++m) {
+      const int e = tid + m * kBlock;
+      const int r = e / Dp, cc = e % Dp;
+      T v = T(0);
+      if (r < rows) {
+        if (cc < d)
+          v = a.Z[(long long)a.order[p0 + r] * d + cc];
+        else if (cc == d)
+          v = T(1);
       }
-      for (int e = tid; e < TR * d; e += kBlock) {
-        const int r = e / d, cc = e - r * d;
-        const int i = idx_s[r];
-        Zs[r * Dp + cc] = (i >= 0) ? a.Z[(long long)i * d + cc] : T(0);
-      }
-      __syncthreads();
-      if (tid < K) {
-        for (int r = 0; r < rows; ++r) csum += (double)Rs[r * Kp + tid];
-      }
-      for (int r = 0; r < rows; ++r) {
-        T rv[KT], zv[DT];
+      zv[m] = v;
+    }
+  };
+
+  int c = c0, r0 = 0;
+  if (c < c1) fetch(c, 0);
+  while (c < c1) {
+    const int seg = a.ch_seg[c];
+    const int rows = min(TR, a.ch_len[c] - r0);
+    __syncthreads();
 #pragma unroll
-        for (int i = 0; i < KT; ++i) rv[i] = Rs[r * Kp + kg * KT + i];
+    for (int m = 0; m < MR; ++m) Rs[tid + m * kBlock] = rv[m];
 #pragma unroll
-        for (int j = 0; j < DT; ++j) zv[j] = Zs[r * Dp + dg * DT + j];
+    for (int m = 0; m < MZ; ++m) Zs[tid + m * kBlock] = zv[m];
+    __syncthreads();
+    // advance the cursor and prefetch the next tile
+    int nc = c, nr0 = r0 + TR;
+    if (nr0 >= a.ch_len[c]) {
+      nc = c + 1;
+      nr0 = 0;
+    }
+    if (nc < c1) fetch(nc, nr0);
+    for (int r = 0; r < rows; ++r) {
+      T rr[KT], zz[DT];
 #pragma unroll
-        for (int i = 0; i < KT; ++i)
+      for (int i = 0; i < KT; ++i) rr[i] = Rs[r * Kp + kg * KT + i];
 #pragma unroll
-          for (int j = 0; j < DT; ++j) acc[i][j] += rv[i] * zv[j];
+      for (int j = 0; j < DT; ++j) zz[j] = Zs[r * Dp + dg * DT + j];
+#pragma unroll
+      for (int i = 0; i < KT; ++i)
+#pragma unroll
+        for (int j = 0; j < DT; ++j) acc[i][j] += rr[i] * zz[j];
+    }
+    // segment finished: flush the ones-column (column sums of R) to acc_in
+    const bool seg_end = (nc >= c1) || (nc != c && a.ch_seg[nc] != seg);
+    if (seg_end) {
+#pragma unroll
+      for (int j = 0; j < DT; ++j) {
+        if (dg * DT + j == d) {
+#pragma unroll
+          for (int i = 0; i < KT; ++i) {
+            const int k = kg * KT + i;
+            if (k < K) fixed_add(&a.acc_in[(long long)seg * K + k], (double)acc[i][j], a.scale_o);
+            acc[i][j] = T(0);
+          }
+        }
       }
     }
-    if (tid < K) fixed_add(&a.acc_in[(long long)seg * K + tid], csum, a.scale_o);
+    c = nc;
+    r0 = nr0;
   }
   T* out = a.Ypart + (long long)blockIdx.x * K * d;
 #pragma unroll
@@ -303,29 +376,62 @@ struct YRedArgs {
   T* Y;
   T* YT;
   int G, K, d, Kp;
-  int* counter;
   EpiArgs<T> epi;
 };
 
-// Y = sum of partials; the last workgroup normalises Y and prepares block 0.
+constexpr int kYRedLds = 40960;
+
+// One workgroup per centroid row k: Y[k] = sum of the G partials (4 waves split
+// the partials, lanes cover the d columns), then the row's L2 normalisation
+// into column k of YT.  Workgroup 0 also prepares block 0 (its old O
+// contribution comes from the centroid pass, an earlier launch).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void yreduce_kernel(YRedArgs<T> a) {
-  __shared__ int s_last;
   __shared__ double red[kBlock / kWave];
+  __shared__ __align__(16) unsigned char smem[kYRedLds];
   if (a.epi.ctrl->done) return;
-  const long long KD = (long long)a.K * a.d;
-  for (long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x; o < KD;
-       o += (long long)gridDim.x * blockDim.x) {
-    double s = 0.0;
-    for (int g = 0; g < a.G; ++g) s += (double)a.Ypart[(long long)g * KD + o];
-    a.Y[o] = T(s);
+  const int k = blockIdx.x;
+  const int d = a.d;
+  const long long KD = (long long)a.K * d;
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  double* part = reinterpret_cast<double*>(smem);  // 4 waves x d
+  for (int c = lane; c < d; c += kWave) {
+    const T* src = a.Ypart + (long long)k * d + c;
+    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int g = w;
+    for (; g + 28 < a.G; g += 32) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s[u] += (double)src[(long long)(g + 4 * u) * KD];
+    }
+    for (; g < a.G; g += 4) s[0] += (double)src[(long long)g * KD];
+    part[w * d + c] = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
   }
-  if (!last_to_arrive(a.counter, &s_last)) return;
-  normalize_centroids(a.Y, a.YT, a.K, a.d, a.Kp);
-  epi_apply(a.epi, a.epi.acc_in, 0, 1);
-  epi_penalty(a.epi);
-  reset_counter(a.counter);
-  (void)red;
+  __syncthreads();
+  // row sum of squares in T, as l2_row_normalize_kernel
+  T ss = T(0);
+  T yv[2] = {T(0), T(0)};
+  if (w == 0) {
+    for (int c = lane, u = 0; c < d; c += kWave, ++u) {
+      double v = part[c] + part[d + c] + part[2 * d + c] + part[3 * d + c];
+      T t = T(v);
+      if (u < 2) yv[u] = t;
+      a.Y[(long long)k * d + c] = t;
+      ss += t * t;
+    }
+    ss = wave_sum(ss);
+    T inv = T(1) / sqrt(ss);
+    if (!(inv <= T(1e12))) inv = T(1e12);
+    for (int c = lane, u = 0; c < d; c += kWave, ++u) a.YT[(long long)c * a.Kp + k] = yv[u] * inv;
+  }
+  if (k == 0) {
+    // zero padding columns K..Kp-1 of YT
+    for (int e = threadIdx.x; e < d * (a.Kp - a.K); e += blockDim.x) {
+      const int c = e / (a.Kp - a.K), kk = a.K + e % (a.Kp - a.K);
+      a.YT[(long long)c * a.Kp + kk] = T(0);
+    }
+    __syncthreads();
+    block_update(a.epi, smem, kYRedLds, 0, 0, kUpdIn | kUpdPen, red);
+  }
 }
 
 // ---------------------------------------------------------------- row kernel
@@ -348,6 +454,7 @@ struct RowArgs {
   T term;
   int n, d, K, Kp;
   int vec_store;
+  int smem_bytes;  // dynamic LDS size (reused by the epilogue)
   double scale_o, scale_obj;
   T* obj_out;  // initialisation: objective written here
   EpiArgs<T> epi;
@@ -378,7 +485,14 @@ __global__ __launch_bounds__(kBlock) void row_kernel(RowArgs<T> a) {
     const int seg = a.ch_seg[c];
     const int b = seg - a.seg_base;
     const int p0 = a.ch_start[c], len = a.ch_len[c];
-    for (int e = tid; e < d * Kp; e += kBlock) YTs[e] = a.YT[e];
+    // normalised centroids: d x Kp contiguous, 16-byte vectors
+    {
+      constexpr int V = 16 / sizeof(T);
+      using VT = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+      const VT* src = reinterpret_cast<const VT*>(a.YT);
+      VT* dst = reinterpret_cast<VT*>(YTs);
+      for (int e = tid; e < d * Kp / V; e += kBlock) dst[e] = src[e];
+    }
     T pv[CT];
 #pragma unroll
     for (int q = 0; q < CT; ++q) {
@@ -484,33 +598,31 @@ __global__ __launch_bounds__(kBlock) void row_kernel(RowArgs<T> a) {
       for (int q = 0; q < CT; ++q) dred[w * Kp + lane * CT + q] = dO[q];
     }
     __syncthreads();
+    unsigned long long sink = 0;
     for (int k = tid; k < K; k += kBlock) {
       double t = 0.0;
       for (int ww = 0; ww < kBlock / kWave; ++ww) t += dred[ww * Kp + k];
-      fixed_add(&a.acc_out[(long long)seg * K + k], t, a.scale_o);
+      sink += fixed_add_wait(&a.acc_out[(long long)seg * K + k], t, a.scale_o);
     }
     km = block_sum(km, red);
     ent = block_sum(ent, red);
     if (tid == 0) {
-      fixed_add(&ctrl->obj_acc[0], km, a.scale_obj);
-      fixed_add(&ctrl->obj_acc[1], ent, a.scale_obj);
+      sink += fixed_add_wait(&ctrl->obj_acc[0], km, a.scale_obj);
+      sink += fixed_add_wait(&ctrl->obj_acc[1], ent, a.scale_obj);
     }
+    asm volatile("" ::"v"(sink));
   }
   if (!last_to_arrive(a.counter, &s_last)) return;
   const EpiArgs<T>& e = a.epi;
   if (a.block < 0) {
     // initialisation: O = scatter(R), E = Pr (x) colsum(R), objective 0
-    epi_apply(e, e.acc_out, 0, 2);
-    T obj = epi_objective(e, red, false);
+    T obj = block_update(e, smem, a.smem_bytes, 0, 0, kUpdOverwrite | kUpdObjInit, red);
     if (tid == 0 && a.obj_out) *a.obj_out = obj;
+  } else if (a.block + 1 < e.nblk) {
+    block_update(e, smem, a.smem_bytes, a.seg_base, a.seg_base + B, kUpdOut | kUpdIn | kUpdPen,
+                 red);
   } else {
-    epi_apply(e, e.acc_out, a.seg_base, 0);
-    if (a.block + 1 < e.nblk) {
-      epi_apply(e, e.acc_in, a.seg_base + B, 1);
-      epi_penalty(e);
-    } else {
-      epi_objective(e, red, true);
-    }
+    block_update(e, smem, a.smem_bytes, a.seg_base, 0, kUpdOut | kUpdObj, red);
   }
   reset_counter(a.counter);
 }
@@ -519,7 +631,8 @@ __global__ __launch_bounds__(kBlock) void row_kernel(RowArgs<T> a) {
 template <typename T>
 __global__ __launch_bounds__(kBlock) void centroids_to_yt_kernel(const T* Y, T* YT, int K, int d,
                                                                  int Kp) {
-  normalize_centroids(Y, YT, K, d, Kp);
+  __shared__ T s_inv[256];
+  normalize_centroids(Y, YT, K, d, Kp, s_inv);
 }
 
 // Control-block reset at the start of a clustering call.

@@ -104,6 +104,23 @@ __device__ __forceinline__ double from_fixed(long long q, double inv_scale) {
   return (double)q * inv_scale;
 }
 
+// Fixed-point add that waits until the device-scope atomic has been performed
+// (its old value is consumed), so a later relaxed arrival counter orders after
+// it without a device-wide release fence.
+__device__ __forceinline__ unsigned long long fixed_add_wait(long long* dst, double v, double scale) {
+  long long q = to_fixed(v, scale);
+  if (q == 0) return 0ull;
+  return __hip_atomic_fetch_add(reinterpret_cast<unsigned long long*>(dst), (unsigned long long)q,
+                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ long long load_agent(const long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_agent(long long* p, long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Number of bits needed for a non-negative value.
 static inline int bit_length(unsigned long long v) {
   int b = 0;

@@ -17,7 +17,8 @@ import numpy as np
 import torch
 
 from ... import _native
-from ._helper import _create_category_index_mapping, _get_batch_codes, _get_theta_array
+from ._helper import _get_batch_codes, _get_theta_array
+from ._job import HarmonyJob
 
 _SUPPRESS_PENALTY = 1e30
 COLSUM_ALGOS = ["columns", "atomics", "gemm", "benchmark", None]
@@ -126,11 +127,9 @@ def harmonize(
     batch_codes = _get_batch_codes(batch_mat, batch_key)
     n_batches = batch_codes.cat.categories.size
     N_b_np = np.asarray(batch_codes.value_counts(sort=False).values, dtype=np_dtype)
-    Pr_b_np = (N_b_np.reshape(-1, 1) / len(batch_codes)).astype(np_dtype)
     cats_np = np.asarray(batch_codes.cat.codes.values, dtype=np.int32)
     if n_cells and cats_np.min() < 0:
         raise ValueError(f"batch key {batch_key!r} has missing values")
-    offsets_np, cell_idx_np = _create_category_index_mapping(cats_np, n_batches)
 
     # parameters (_harmony/__init__.py:155-200)
     if max_iter_harmony < 1:
@@ -167,67 +166,29 @@ def harmonize(
 
     if not Z.is_cuda:
         raise TypeError("harmonize needs Z on a HIP device (torch 'cuda' tensor)")
-    Z_norm = l2_normalize(Z)
-    tdt = Z.dtype
-    cats = torch.as_tensor(cats_np, device=dev)
-    cat_offsets = torch.as_tensor(offsets_np, device=dev)
-    cell_indices = torch.as_tensor(cell_idx_np, device=dev)
-    N_b = torch.as_tensor(N_b_np, device=dev)
-    Pr_b = torch.as_tensor(Pr_b_np.ravel(), device=dev)
-    theta_t = torch.as_tensor(theta_np, device=dev)
-
-    lib = _native.load()
-    dims = _native.dims(n_cells, n_pcs, n_clusters, n_batches, block_size, dsz, prec)
-    nbytes = lib.hz_workspace_bytes(ctypes.byref(dims))
-    if nbytes == 0:
-        msg = lib.hz_last_error().decode()
-        raise ValueError(f"unsupported Harmony shape: {msg}")
-    ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
-    stream = _stream_handle(dev)
+    job = HarmonyJob(
+        Z, cats_np, n_batches, n_clusters=n_clusters, theta_np=theta_np, N_b_np=N_b_np,
+        sigma=sigma, block_proportion=block_proportion, max_iter_clustering=max_iter_clustering,
+        tol_clustering=tol_clustering, ridge_lambda=ridge_lambda,
+        stabilized_penalty=stabilized_penalty, dynamic_lambda=dynamic_lambda, alpha=alpha,
+        batch_prune_threshold=batch_prune_threshold, correction_method=correction_method,
+        random_state=random_state, precision=prec)
 
     # initialisation (_harmony/__init__.py:206-217, 292-353)
-    if init_centroids is None:
-        Y0 = kmeans_centroids(Z_norm, n_clusters, random_state)
-    else:
-        Y0 = _to_device(init_centroids, dev, tdt)
+    Y0 = None
+    if init_centroids is not None:
+        Y0 = _to_device(init_centroids, dev, Z.dtype)
         if tuple(Y0.shape) != (n_clusters, n_pcs):
             raise ValueError(f"init_centroids must have shape {(n_clusters, n_pcs)}")
-    R = torch.empty((n_cells, n_clusters), dtype=tdt, device=dev)
-    E = torch.empty((n_batches, n_clusters), dtype=tdt, device=dev)
-    O = torch.empty((n_batches, n_clusters), dtype=tdt, device=dev)
-    obj = torch.empty(1, dtype=tdt, device=dev)
-    rc = lib.hz_init_clusters(ctypes.byref(dims), _ptr(Z_norm), _ptr(Y0), _ptr(cats),
-                              _ptr(cat_offsets), _ptr(cell_indices), _ptr(Pr_b), _ptr(theta_t),
-                              float(sigma), int(bool(stabilized_penalty)), _ptr(R), _ptr(E),
-                              _ptr(O), _ptr(obj), _ptr(ws), nbytes, stream)
-    _native.check(rc, "hz_init_clusters")
-    objectives_harmony = [float(obj.item())]
+    job.init(Y0)
+    objectives_harmony = [float(job.obj.item())]
 
-    method = _native.HZ_CORR[correction_method]
-    has_thr = batch_prune_threshold is not None
-    thr = float(batch_prune_threshold) if has_thr else 0.0
-    Z_hat = torch.empty_like(Z)
-    lam = torch.empty((n_batches, n_clusters), dtype=tdt, device=dev)
-    last_obj = torch.zeros(1, dtype=tdt, device=dev)
-    n_iter = ctypes.c_int(0)
     clustering_iters = []
     is_converged = False
     for i in range(max_iter_harmony):
-        seed = (random_state + i * 1000003) & _U32
-        rc = lib.hz_clustering_loop(ctypes.byref(dims), _ptr(Z_norm), _ptr(R), _ptr(E), _ptr(O),
-                                    _ptr(Pr_b), _ptr(cats), _ptr(theta_t), float(sigma),
-                                    float(tol_clustering), int(max_iter_clustering), seed,
-                                    int(bool(stabilized_penalty)), _ptr(last_obj),
-                                    ctypes.byref(n_iter), _ptr(ws), nbytes, stream)
-        _native.check(rc, "hz_clustering_loop")
-        clustering_iters.append(n_iter.value)
-        objectives_harmony.append(float(last_obj.item()))
-        rc = lib.hz_correction(ctypes.byref(dims), _ptr(Z), _ptr(R), _ptr(O), _ptr(E), _ptr(N_b),
-                               _ptr(cat_offsets), _ptr(cell_indices), method,
-                               int(bool(dynamic_lambda)), float(alpha), int(has_thr), thr,
-                               float(ridge_lambda), _ptr(lam), _ptr(Z_hat), _ptr(Z_norm),
-                               _ptr(ws), nbytes, stream)
-        _native.check(rc, "hz_correction")
+        clustering_iters.append(job.clustering(i))
+        objectives_harmony.append(float(job.last_obj.item()))
+        job.correction()
         if _is_convergent_harmony(objectives_harmony, tol=tol_harmony):
             is_converged = True
             if verbose:
@@ -236,10 +197,10 @@ def harmonize(
     if not is_converged:
         warnings.warn("Harmony did not converge. Consider increasing the number of iterations")
     if record is not None:
-        record.update(objectives=objectives_harmony, clustering_iters=clustering_iters, R=R, E=E,
-                      O=O, Z_norm=Z_norm, lambda_kb=lam, Y0=Y0, n_clusters=n_clusters,
-                      correction_method=correction_method)
-    return Z_hat
+        record.update(objectives=objectives_harmony, clustering_iters=clustering_iters, R=job.R,
+                      E=job.E, O=job.O, Z_norm=job.Z_norm, lambda_kb=job.lam, Y0=job.Y0,
+                      n_clusters=n_clusters, correction_method=correction_method)
+    return job.Z_hat
 
 
 def _is_convergent_harmony(objectives_harmony: list, tol: float) -> bool:
