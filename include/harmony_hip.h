@@ -39,7 +39,7 @@ extern "C" {
 #define HZ_F64 8
 
 #define HZ_PREC_NATIVE 0     /* GEMMs in the storage type */
-#define HZ_PREC_SPLIT_BF16 1 /* FP32 storage, split-BF16 products (reserved) */
+#define HZ_PREC_SPLIT_BF16 1 /* FP32 storage, split-BF16 matrix-core products */
 
 #define HZ_CORR_BATCHED 0
 #define HZ_CORR_FAST 1
@@ -74,6 +74,16 @@ const char* hz_last_error(void);
 #define HZ_PROF_KINDS 8
 int hz_profile_enable(int on);
 int hz_profile_collect(double* ms_out, long long* launches_out, int n_kinds);
+
+/* Process-wide tuning knobs (0 = automatic choice), read when a workspace
+ * layout is computed, so a workspace must be sized and used under the same
+ * settings.  Used by the autotuner; no counterpart in the reference (its
+ * closest analogue is the colsum_algo benchmark, _harmony/_helper.py:401-504). */
+#define HZ_TUNE_ROW_CHUNK 0   /* rows per chunk of the row kernel (multiple of 16) */
+#define HZ_TUNE_CENTROID_WG 1 /* workgroups of the centroid pass */
+#define HZ_TUNE_KEYS 4
+int hz_set_tuning(int key, int value);
+int hz_get_tuning(int key);
 
 /* Workspace for hz_init_clusters / hz_clustering_loop / hz_correction.
  * Replaces _allocate_clustering_workspace (_harmony/__init__.py:356-384),

@@ -19,6 +19,7 @@ HZ_F64 = 8
 HZ_PREC_NATIVE = 0
 HZ_PREC_SPLIT_BF16 = 1
 HZ_CORR = {"batched": 0, "fast": 1, "original": 2}
+TUNE_KEYS = {"row_chunk": 0, "centroid_wg": 1}
 PROF_KINDS = ["row", "centroid", "yreduce", "order", "correction", "phi", "apply", "unused"]
 
 _vp = ctypes.c_void_p
@@ -47,6 +48,8 @@ _P = ctypes.POINTER(HzDims)
 _SIGNATURES = {
     "hz_version": (_i, []),
     "hz_profile_enable": (_i, [_i]),
+    "hz_set_tuning": (_i, [_i, _i]),
+    "hz_get_tuning": (_i, [_i]),
     "hz_profile_collect": (_i, [ctypes.POINTER(_d), ctypes.POINTER(_ll), _i]),
     "hz_last_error": (ctypes.c_char_p, []),
     "hz_workspace_bytes": (_sz, [_P]),
@@ -137,3 +140,15 @@ def profile_collect() -> dict:
     cnt = (_ll * len(PROF_KINDS))()
     check(lib.hz_profile_collect(ms, cnt, len(PROF_KINDS)), "hz_profile_collect")
     return {k: (ms[i], cnt[i]) for i, k in enumerate(PROF_KINDS)}
+
+
+def set_tuning(**kw) -> None:
+    """Set library tuning knobs (0 = automatic): row_chunk, centroid_wg."""
+    lib = load()
+    for k, v in kw.items():
+        check(lib.hz_set_tuning(TUNE_KEYS[k], int(v)), f"hz_set_tuning({k})")
+
+
+def get_tuning() -> dict:
+    lib = load()
+    return {k: lib.hz_get_tuning(i) for k, i in TUNE_KEYS.items()}

@@ -3,9 +3,11 @@
 
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "hz_correct.hpp"
+#include "hz_mfma.hpp"
 
 namespace hz {
 
@@ -91,14 +93,14 @@ struct ProfScope {
   ProfScope(int k, hipStream_t st) : s(st), kind(k) {
     if (g_prof) {
       a = prof_event();
-      if (a) hipEventRecord(a, s);
+      if (a) (void)hipEventRecord(a, s);
     }
   }
   ~ProfScope() {
     if (g_prof && a) {
       hipEvent_t b = prof_event();
       if (b) {
-        hipEventRecord(b, s);
+        (void)hipEventRecord(b, s);
         g_prof_recs.push_back({a, b, kind});
       }
     }
@@ -139,10 +141,17 @@ static int fixed_shift(unsigned long long max_abs) {
 
 // ---------------------------------------------------------------- layout
 
+// Tuning knobs (0 = automatic), set through hz_set_tuning.
+static int g_tune[HZ_TUNE_KEYS];
+
 struct Layout {
   int n, d, K, B, bs, esz;
   int CT, Kp, DT, Dp;
   int chunk, chunk_p, G, cpw;
+  // matrix-core path (FP32 storage)
+  int mfma, prec, NT, NU, dP, cent_mfma, Gm, cpwm;
+  void* YH;
+  void* YL;
   PermDims pd;
   PermDims sd;  // static row chunks (cells grouped by batch)
   PermDims spd; // static phi chunks
@@ -155,7 +164,8 @@ struct Layout {
   ChunkTab sp;
   int* nch_tmp_p;
   long long* acc_in;
-  long long* acc_out;
+  long long* acc_out;  // kRep replicas
+  long long* obj_acc;  // kRep x 2
   Ctrl* ctrl;
   Ctrl* ctrl_static;  // never set done: used by the correction pass
   int* counters;      // nblk + 4
@@ -181,8 +191,10 @@ static int make_layout(const hz_dims* dm, void* base, Layout& L) {
             dm->n_cells, dm->n_pcs, dm->n_clusters, dm->n_batches);
   HZ_RET_IF(dm->block_size < 1, HZ_ERR_ARG,
             "block_size must be >= 1 (int(n_cells * block_proportion) = %d)", dm->block_size);
-  HZ_RET_IF(dm->precision != HZ_PREC_NATIVE, HZ_ERR_UNSUPPORTED,
-            "precision %d is not available in this build", dm->precision);
+  HZ_RET_IF(dm->precision != HZ_PREC_NATIVE && dm->precision != HZ_PREC_SPLIT_BF16, HZ_ERR_ARG,
+            "unknown precision %d", dm->precision);
+  HZ_RET_IF(dm->precision == HZ_PREC_SPLIT_BF16 && dm->dtype != HZ_F32, HZ_ERR_ARG,
+            "split-BF16 precision needs float32 data");
   L.n = dm->n_cells;
   L.d = dm->n_pcs;
   L.K = dm->n_clusters;
@@ -200,7 +212,26 @@ static int make_layout(const hz_dims* dm, void* base, Layout& L) {
   }
   L.Kp = 16 * L.CT;
   L.Dp = 16 * L.DT;
-  L.chunk = (L.bs <= 32768) ? 64 : 128;
+  // matrix cores for FP32 storage: NT 16-column tiles of K, NU tiles of d+1
+  L.prec = dm->precision;
+  // matrix-core kernels index with 32-bit element offsets
+  L.mfma = (L.esz == 4 && (long long)L.n * std::max(L.K, L.d + 1) < (1LL << 31)) ? 1 : 0;
+  HZ_RET_IF(L.prec == HZ_PREC_SPLIT_BF16 && !L.mfma, HZ_ERR_UNSUPPORTED,
+            "split-BF16 precision supports at most 2^31 matrix elements per operand");
+  {
+    const int nt = (int)ceil_div(L.K, 16), nu = (int)ceil_div(L.d + 1, 16);
+    L.NT = nt <= 4 ? 4 : nt <= 7 ? 7 : nt <= 8 ? 8 : 16;
+    L.NU = nu <= 2 ? 2 : nu <= 3 ? 3 : nu <= 4 ? 4 : nu <= 7 ? 7 : 8;
+    L.cent_mfma = L.mfma && (L.NT * L.NU <= 32);
+  }
+  L.dP = (int)ceil_div(L.d, 32) * 32;  // = 32 * row k-steps in split-BF16 mode
+  if (g_tune[HZ_TUNE_ROW_CHUNK] > 0)
+    L.chunk = g_tune[HZ_TUNE_ROW_CHUNK];
+  else if (L.mfma)
+    L.chunk = (L.bs <= 32768) ? 16 : 32;
+  else
+    L.chunk = (L.bs <= 32768) ? 64 : 128;
+  if (L.mfma) L.chunk = std::min(L.chunk, 64);  // a wave holds a chunk's cell ids
   {
     long long target = ceil_div(L.n, 512);
     int cp = 64;
@@ -208,11 +239,20 @@ static int make_layout(const hz_dims* dm, void* base, Layout& L) {
     L.chunk_p = cp;
   }
   L.pd = perm_dims(L.n, L.bs, L.B, L.chunk);
-  L.sd = perm_dims(L.n, L.n, L.B, L.chunk);
+  L.sd = perm_dims(L.n, L.n, L.B, 64);  // static batch-grouped table: 64-row chunks
   L.spd = perm_dims(L.n, L.n, L.B, L.chunk_p);
   L.G = (int)std::min<long long>(512, L.pd.maxch);
   L.cpw = (int)ceil_div(L.pd.maxch, L.G);
   L.G = (int)ceil_div(L.pd.maxch, L.cpw);
+  if (L.cent_mfma) {
+    // centroid pass on matrix cores: a contiguous chunk range per workgroup
+    long long gm = g_tune[HZ_TUNE_CENTROID_WG] > 0 ? g_tune[HZ_TUNE_CENTROID_WG]
+                                                   : 2LL * device_cu_count();
+    gm = std::max<long long>(1, std::min<long long>(gm, L.pd.maxch));
+    L.cpwm = (int)ceil_div(L.pd.maxch, gm);
+    L.Gm = (int)ceil_div(L.pd.maxch, (long long)L.cpwm);
+    L.G = std::max(L.G, L.Gm);
+  }
   const long long nseg = (long long)L.pd.nblk * L.B;
   const long long BK = (long long)L.B * L.K;
   const long long KD = (long long)L.K * L.d;
@@ -239,13 +279,16 @@ static int make_layout(const hz_dims* dm, void* base, Layout& L) {
     L.nch_tmp_p = c.take<int>(L.B + 1);
   }
   L.acc_in = c.take<long long>(nseg * L.K);
-  L.acc_out = c.take<long long>(nseg * L.K);
+  L.acc_out = c.take<long long>((long long)kRep * nseg * L.K);
+  L.obj_acc = c.take<long long>(2 * kRep);
   L.ctrl = c.take<Ctrl>(1);
   L.ctrl_static = c.take<Ctrl>(1);
   L.counters = c.take<int>(L.pd.nblk + 4);
   L.Ypart = c.take<char>((size_t)L.G * KD * L.esz);
   L.Y = c.take<char>((size_t)KD * L.esz);
   L.YT = c.take<char>((size_t)L.d * L.Kp * L.esz);
+  L.YH = c.take<char>((size_t)L.Kp * L.dP * 2);
+  L.YL = c.take<char>((size_t)L.Kp * L.dP * 2);
   L.pen = c.take<char>((size_t)BK * L.esz);
   L.phipart = c.take<char>((size_t)L.spd.maxch * KD * L.esz);
   L.Phi = c.take<double>(BK * L.d);
@@ -269,6 +312,8 @@ static EpiArgs<T> make_epi(const Layout& L, T* O, T* E, const T* Pr, const T* th
   e.pen = reinterpret_cast<T*>(L.pen);
   e.acc_in = L.acc_in;
   e.acc_out = L.acc_out;
+  e.obj_acc = L.obj_acc;
+  e.rep_stride = (long long)L.pd.nblk * L.B * L.K;
   e.ctrl = L.ctrl;
   e.last_obj = last_obj;
   e.B = L.B;
@@ -346,6 +391,142 @@ static int launch_centroid(const Layout& L, const CentArgs<T>& a, int grid, hipS
   }
 }
 
+// ---------------------------------------------------------------- matrix-core dispatch
+
+template <int PREC, int NT, int SM>
+static int launch_rm(RowMArgs a, int grid, size_t lds, hipStream_t s) {
+  auto kern = row_mfma_kernel<PREC, NT, SM>;
+  if (lds > 65536)
+    HZ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), lds, s, a);
+  HZ_LAUNCHED("row_mfma_kernel");
+  return HZ_OK;
+}
+
+// k-steps of the row kernel: f32 4 columns per step, bf16 32 per step
+static int row_sm(const Layout& L) {
+  if (L.prec == HZ_PREC_SPLIT_BF16) return (int)ceil_div(L.d, 32);  // 1..4
+  const int s = (int)ceil_div(L.d, 4);
+  return s <= 4 ? 4 : s <= 8 ? 8 : s <= 13 ? 13 : s <= 16 ? 16 : s <= 25 ? 25 : 32;
+}
+
+template <int PREC, int SM>
+static int launch_rm_sm(const Layout& L, RowMArgs a, int grid, hipStream_t s) {
+  const size_t lds = std::max(row_mfma_lds<PREC, SM>(L.Kp), (size_t)block_update_min_lds<float>(L.B));
+  HZ_RET_IF(lds > (size_t)device_lds_limit(), HZ_ERR_UNSUPPORTED,
+            "row kernel needs %zu bytes of LDS (limit %d)", lds, device_lds_limit());
+  a.smem_bytes = (int)lds;
+  switch (L.NT) {
+    case 4: return launch_rm<PREC, 4, SM>(a, grid, lds, s);
+    case 7: return launch_rm<PREC, 7, SM>(a, grid, lds, s);
+    case 8: return launch_rm<PREC, 8, SM>(a, grid, lds, s);
+    default: return launch_rm<PREC, 16, SM>(a, grid, lds, s);
+  }
+}
+
+template <int PREC>
+static int launch_row_mfma_p(const Layout& L, RowMArgs a, int grid, hipStream_t s) {
+  if constexpr (PREC == kPrecBF16) {
+    switch (row_sm(L)) {
+      case 1: return launch_rm_sm<PREC, 1>(L, a, grid, s);
+      case 2: return launch_rm_sm<PREC, 2>(L, a, grid, s);
+      case 3: return launch_rm_sm<PREC, 3>(L, a, grid, s);
+      default: return launch_rm_sm<PREC, 4>(L, a, grid, s);
+    }
+  } else {
+    switch (row_sm(L)) {
+      case 4: return launch_rm_sm<PREC, 4>(L, a, grid, s);
+      case 8: return launch_rm_sm<PREC, 8>(L, a, grid, s);
+      case 13: return launch_rm_sm<PREC, 13>(L, a, grid, s);
+      case 16: return launch_rm_sm<PREC, 16>(L, a, grid, s);
+      case 25: return launch_rm_sm<PREC, 25>(L, a, grid, s);
+      default: return launch_rm_sm<PREC, 32>(L, a, grid, s);
+    }
+  }
+}
+
+static int launch_row_mfma(const Layout& L, const RowMArgs& a, int grid, hipStream_t s) {
+  if (L.prec == HZ_PREC_SPLIT_BF16) return launch_row_mfma_p<kPrecBF16>(L, a, grid, s);
+  return launch_row_mfma_p<kPrecF32>(L, a, grid, s);
+}
+
+template <int PREC, int NT, int NU>
+static int launch_cm(const CentMArgs& a, int grid, hipStream_t s) {
+  if constexpr (NT * NU <= 32) {
+    auto kern = centroid_mfma_kernel<PREC, NT, NU>;
+    const size_t lds = centroid_tile_lds<PREC, NT, NU>();
+    HZ_RET_IF(lds > (size_t)device_lds_limit(), HZ_ERR_UNSUPPORTED,
+              "centroid pass needs %zu bytes of LDS", lds);
+    if (lds > 65536)
+      HZ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), lds, s, a);
+    HZ_LAUNCHED("centroid_mfma_kernel");
+    return HZ_OK;
+  } else {
+    set_error("centroid tile shape %d x %d not instantiated", NT, NU);
+    return HZ_ERR_UNSUPPORTED;
+  }
+}
+
+template <int PREC, int NT>
+static int launch_cm_nt(const Layout& L, const CentMArgs& a, int grid, hipStream_t s) {
+  switch (L.NU) {
+    case 2: return launch_cm<PREC, NT, 2>(a, grid, s);
+    case 3: return launch_cm<PREC, NT, 3>(a, grid, s);
+    case 4: return launch_cm<PREC, NT, 4>(a, grid, s);
+    case 7: return launch_cm<PREC, NT, 7>(a, grid, s);
+    default: return launch_cm<PREC, NT, 8>(a, grid, s);
+  }
+}
+
+template <int PREC>
+static int launch_cent_mfma_p(const Layout& L, const CentMArgs& a, int grid, hipStream_t s) {
+  switch (L.NT) {
+    case 4: return launch_cm_nt<PREC, 4>(L, a, grid, s);
+    case 7: return launch_cm_nt<PREC, 7>(L, a, grid, s);
+    case 8: return launch_cm_nt<PREC, 8>(L, a, grid, s);
+    default: return launch_cm_nt<PREC, 16>(L, a, grid, s);
+  }
+}
+
+static int launch_cent_mfma(const Layout& L, const CentMArgs& a, int grid, hipStream_t s) {
+  if (L.prec == HZ_PREC_SPLIT_BF16) return launch_cent_mfma_p<kPrecBF16>(L, a, grid, s);
+  return launch_cent_mfma_p<kPrecF32>(L, a, grid, s);
+}
+
+static int launch_split(const Layout& L, const Ctrl* ctrl, hipStream_t s) {
+  if (L.prec != HZ_PREC_SPLIT_BF16) return HZ_OK;
+  hipLaunchKernelGGL(centroid_split_kernel, dim3(L.Kp), dim3(64), 0, s,
+                     reinterpret_cast<const float*>(L.YT), reinterpret_cast<__bf16*>(L.YH),
+                     reinterpret_cast<__bf16*>(L.YL), L.K, L.Kp, L.d, L.dP, ctrl);
+  HZ_LAUNCHED("centroid_split_kernel");
+  return HZ_OK;
+}
+
+template <typename T>
+static RowMArgs make_row_margs(const Layout& L, const T* Z, T* R, const EpiArgs<T>& epi) {
+  RowMArgs a{};
+  if constexpr (std::is_same<T, float>::value) {
+    a.Z = Z;
+    a.R = R;
+    a.epi = epi;
+  }
+  a.YT = reinterpret_cast<const float*>(L.YT);
+  a.YH = reinterpret_cast<const __bf16*>(L.YH);
+  a.YL = reinterpret_cast<const __bf16*>(L.YL);
+  a.acc_out = L.acc_out;
+  a.n = L.n;
+  a.d = L.d;
+  a.K = L.K;
+  a.Kp = L.Kp;
+  a.dP = L.dP;
+  a.scale_o = L.scale_o;
+  a.scale_obj = L.scale_obj;
+  return a;
+}
+
 static uint32_t* g_pinned_flag = nullptr;
 
 static int read_done(const Ctrl* ctrl, hipStream_t s, int* done) {
@@ -368,10 +549,11 @@ static int init_impl(const hz_dims* dm, const T* Z, const T* Y0, const int* cats
   HZ_TRY(check_ws(L, ws, ws_bytes));
   HZ_TRY(make_layout(dm, ws, L));
   (void)cats;
-  HZ_TRY(launch_static_chunks(L.n, L.B, L.chunk, cat_offsets, cell_indices, L.st, L.nch_tmp_s, s));
+  HZ_TRY(launch_static_chunks(L.n, L.B, L.sd.chunk, cat_offsets, cell_indices, L.st, L.nch_tmp_s, s));
   hipLaunchKernelGGL(ctrl_reset_kernel, dim3(1), dim3(64), 0, s, L.ctrl);
   HZ_LAUNCHED("ctrl_reset");
-  HZ_TRY(fill<long long>(L.acc_out, (long long)L.B * L.K, 0, s));
+  HZ_TRY(fill<long long>(L.acc_out, (long long)kRep * L.pd.nblk * L.B * L.K, 0, s));
+  HZ_TRY(fill<long long>(L.obj_acc, 2 * kRep, 0, s));
   HZ_TRY(fill<int>(L.counters, L.pd.nblk + 4, 0, s));
   hipLaunchKernelGGL(centroids_to_yt_kernel<T>, dim3(1), dim3(kBlock), 0, s, Y0,
                      reinterpret_cast<T*>(L.YT), L.K, L.d, L.Kp);
@@ -400,6 +582,24 @@ static int init_impl(const hz_dims* dm, const T* Z, const T* Y0, const int* cats
   a.scale_obj = L.scale_obj;
   a.obj_out = obj_out;
   a.epi = make_epi<T>(L, O, E, Pr, theta, sigma, 0.0, stabilized, nullptr);
+  if constexpr (std::is_same<T, float>::value) {
+    if (L.mfma) {
+      HZ_TRY(launch_split(L, nullptr, s));
+      RowMArgs m = make_row_margs<T>(L, Z, R, a.epi);
+      m.pen = nullptr;
+      m.order = L.st.order;
+      m.seg_ch = L.st.seg_ch;
+      m.ch_start = L.st.ch_start;
+      m.ch_len = L.st.ch_len;
+      m.ch_seg = L.st.ch_seg;
+      m.counter = a.counter;
+      m.seg_base = 0;
+      m.block = -1;
+      m.term = a.term;
+      m.obj_out = obj_out;
+      return launch_row_mfma(L, m, (int)ceil_div(L.sd.maxch, kBlock / kWave), s);
+    }
+  }
   const int grid = (int)L.sd.maxch;
   return launch_row<T>(L, a, grid, s);
 }
@@ -421,8 +621,7 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
   const long long nseg = (long long)L.pd.nblk * L.B;
   hipLaunchKernelGGL(ctrl_reset_kernel, dim3(1), dim3(64), 0, s, L.ctrl);
   HZ_LAUNCHED("ctrl_reset");
-  HZ_TRY(fill<long long>(L.acc_in, nseg * L.K, 0, s));
-  HZ_TRY(fill<long long>(L.acc_out, nseg * L.K, 0, s));
+  // acc_out / obj_acc are cleared by the centroid reduction of every iteration
   HZ_TRY(fill<int>(L.counters, L.pd.nblk + 4, 0, s));
   if (last_obj) HZ_TRY(fill<T>(last_obj, 1, T(0), s));
 
@@ -478,6 +677,42 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
   ra.obj_out = nullptr;
   ra.epi = epi;
 
+  // matrix-core variants (FP32 storage)
+  bool use_rm = false, use_cm = false;
+  RowMArgs rm{};
+  CentMArgs cm{};
+  if constexpr (std::is_same<T, float>::value) {
+    use_rm = L.mfma != 0;
+    use_cm = L.cent_mfma != 0;
+    rm = make_row_margs<T>(L, Z, R, epi);
+    rm.pen = reinterpret_cast<const float*>(L.pen);
+    rm.order = L.ct.order;
+    rm.seg_ch = L.ct.seg_ch;
+    rm.ch_start = L.ct.ch_start;
+    rm.ch_len = L.ct.ch_len;
+    rm.ch_seg = L.ct.ch_seg;
+    rm.term = ra.term;
+    rm.obj_out = nullptr;
+    cm.Z = Z;
+    cm.R = R;
+    cm.order = L.ct.order;
+    cm.seg_start = L.ct.seg_start;
+    cm.seg_ch = L.ct.seg_ch;
+    cm.ch_start = L.ct.ch_start;
+    cm.ch_len = L.ct.ch_len;
+    cm.ch_seg = L.ct.ch_seg;
+    cm.acc_in = L.acc_in;
+    cm.Ypart = reinterpret_cast<float*>(L.Ypart);
+    cm.ctrl = L.ctrl;
+    cm.nseg = (int)nseg;
+    cm.cpw = L.cpwm;
+    cm.n = L.n;
+    cm.d = L.d;
+    cm.K = L.K;
+    cm.scale_o = L.scale_o;
+  }
+  if (use_cm) ya.G = L.Gm;
+
   const int check_every = 4;
   for (int it = 0; it < max_iter; ++it) {
     if (it > 0 && it % check_every == 0) {
@@ -490,23 +725,35 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
       HZ_TRY(launch_block_order(L.pd, seed + (uint32_t)it, cats, L.pb, L.ct, L.nch_tmp,
                                 &L.ctrl->done, s));
     }
+    HZ_TRY(fill<long long>(L.acc_in, nseg * L.K, 0, s));
     {
       ProfScope ps(HZ_PROF_CENTROID, s);
-      HZ_TRY(launch_centroid<T>(L, ca, L.G, s));
+      if (use_cm)
+        HZ_TRY(launch_cent_mfma(L, cm, L.Gm, s));
+      else
+        HZ_TRY(launch_centroid<T>(L, ca, L.G, s));
     }
     {
       ProfScope ps(HZ_PROF_YREDUCE, s);
       hipLaunchKernelGGL(yreduce_kernel<T>, dim3(ygrid), dim3(kBlock), 0, s, ya);
       HZ_LAUNCHED("yreduce_kernel");
+      if (use_rm) HZ_TRY(launch_split(L, L.ctrl, s));
     }
     for (int j = 0; j < L.pd.nblk; ++j) {
       const int rows_j = std::min(L.bs, L.n - j * L.bs);
       const int grid = (int)(ceil_div(rows_j, L.chunk) + L.B);
-      ra.seg_base = j * L.B;
-      ra.block = j;
-      ra.counter = L.counters + j;
       ProfScope ps(HZ_PROF_ROW, s);
-      HZ_TRY(launch_row<T>(L, ra, grid, s));
+      if (use_rm) {
+        rm.seg_base = j * L.B;
+        rm.block = j;
+        rm.counter = L.counters + j;
+        HZ_TRY(launch_row_mfma(L, rm, (int)ceil_div(grid, kBlock / kWave), s));
+      } else {
+        ra.seg_base = j * L.B;
+        ra.block = j;
+        ra.counter = L.counters + j;
+        HZ_TRY(launch_row<T>(L, ra, grid, s));
+      }
     }
   }
   Ctrl h;
@@ -547,7 +794,7 @@ static int correction_impl(const hz_dims* dm, const T* X, const T* R, const T* O
   HZ_LAUNCHED("lambda_kernel");
   // static chunk tables (cells grouped by batch)
   HZ_TRY(launch_static_chunks(L.n, L.B, L.chunk_p, cat_offsets, cell_indices, L.sp, L.nch_tmp_p, s));
-  HZ_TRY(launch_static_chunks(L.n, L.B, L.chunk, cat_offsets, cell_indices, L.st, L.nch_tmp_s, s));
+  HZ_TRY(launch_static_chunks(L.n, L.B, L.sd.chunk, cat_offsets, cell_indices, L.st, L.nch_tmp_s, s));
   hipLaunchKernelGGL(ctrl_reset_kernel, dim3(1), dim3(64), 0, s, L.ctrl_static);
   HZ_LAUNCHED("ctrl_reset");
   HZ_TRY(fill<long long>(L.rsum, BK, 0, s));
@@ -811,6 +1058,17 @@ using namespace hz;
 extern "C" {
 
 int hz_version(void) { return 100; }
+
+int hz_set_tuning(int key, int value) {
+  HZ_RET_IF(key < 0 || key >= HZ_TUNE_KEYS, HZ_ERR_ARG, "unknown tuning key %d", key);
+  HZ_RET_IF(value < 0, HZ_ERR_ARG, "tuning value must be >= 0");
+  HZ_RET_IF(key == HZ_TUNE_ROW_CHUNK && value % 16 != 0, HZ_ERR_ARG,
+            "row chunk must be a multiple of 16");
+  g_tune[key] = value;
+  return HZ_OK;
+}
+
+int hz_get_tuning(int key) { return (key >= 0 && key < HZ_TUNE_KEYS) ? g_tune[key] : -1; }
 
 int hz_profile_enable(int on) {
   g_prof = on != 0;

@@ -31,8 +31,13 @@ struct Ctrl {
   int pad0;
   int pad1;
   double hist[4];
-  long long obj_acc[2];  // k-means error, entropy (fixed point)
 };
+
+// Block contributions to O and the objective terms are added by many
+// workgroups at once; spreading them over kRep replicas (replica = workgroup
+// id mod kRep) keeps same-address atomics from serialising.  Readers sum the
+// replicas.
+constexpr int kRep = 8;
 
 template <typename T>
 struct EpiArgs {
@@ -42,7 +47,9 @@ struct EpiArgs {
   const T* theta;
   T* pen;              // B x K penalty for the next block
   long long* acc_in;   // nseg x K
-  long long* acc_out;  // nseg x K
+  long long* acc_out;  // kRep x nseg x K
+  long long* obj_acc;  // kRep x 2: k-means error, entropy (fixed point)
+  long long rep_stride;  // nseg x K
   Ctrl* ctrl;
   T* last_obj;         // device scalar written after every iteration
   int B, K, nblk;
@@ -98,14 +105,15 @@ __device__ T block_update(const EpiArgs<T>& a, unsigned char* smem, int smem_byt
       const long long p = (long long)b * K + k0 + kk;
       double vo = 0.0, vi = 0.0;
       if (flags & (kUpdOut | kUpdOverwrite)) {
-        long long* src = a.acc_out + (long long)(seg_out + b) * K + k0 + kk;
-        vo = (double)load_agent(src) * inv;
-        store_agent(src, 0);
+        const long long* src = a.acc_out + (long long)(seg_out + b) * K + k0 + kk;
+        long long q = 0;
+#pragma unroll
+        for (int r = 0; r < kRep; ++r) q += load_agent(src + r * a.rep_stride);
+        vo = (double)q * inv;
       }
       if (flags & kUpdIn) {
-        long long* src = a.acc_in + (long long)(seg_in + b) * K + k0 + kk;
-        vi = (double)load_agent(src) * inv;
-        store_agent(src, 0);
+        const long long* src = a.acc_in + (long long)(seg_in + b) * K + k0 + kk;
+        vi = (double)(*src) * inv;  // written by an earlier launch
       }
       s_do[q] = vo;
       s_di[q] = vi;
@@ -160,10 +168,13 @@ __device__ T block_update(const EpiArgs<T>& a, unsigned char* smem, int smem_byt
   __shared__ T s_obj;
   if (tid == 0) {
     Ctrl* c = a.ctrl;
-    double km = (double)load_agent(&c->obj_acc[0]) * a.inv_scale_obj;
-    double ent = (double)load_agent(&c->obj_acc[1]) * a.inv_scale_obj;
-    store_agent(&c->obj_acc[0], 0);
-    store_agent(&c->obj_acc[1], 0);
+    long long qk = 0, qe = 0;
+    for (int r = 0; r < kRep; ++r) {
+      qk += load_agent(&a.obj_acc[2 * r]);
+      qe += load_agent(&a.obj_acc[2 * r + 1]);
+    }
+    double km = (double)qk * a.inv_scale_obj;
+    double ent = (double)qe * a.inv_scale_obj;
     T obj = T(km + (double)a.sigma * ent + (double)a.sigma * div);
     s_obj = obj;
     if (flags & kUpdObj) {
@@ -390,6 +401,14 @@ __global__ __launch_bounds__(kBlock) void yreduce_kernel(YRedArgs<T> a) {
   __shared__ double red[kBlock / kWave];
   __shared__ __align__(16) unsigned char smem[kYRedLds];
   if (a.epi.ctrl->done) return;
+  // clear this iteration's block accumulators (read by earlier launches only)
+  {
+    const long long n_out = (long long)kRep * a.epi.rep_stride;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n_out;
+         e += (long long)gridDim.x * blockDim.x)
+      a.epi.acc_out[e] = 0;
+    if (blockIdx.x == 0 && threadIdx.x < 2 * kRep) a.epi.obj_acc[threadIdx.x] = 0;
+  }
   const int k = blockIdx.x;
   const int d = a.d;
   const long long KD = (long long)a.K * d;
@@ -598,17 +617,18 @@ __global__ __launch_bounds__(kBlock) void row_kernel(RowArgs<T> a) {
       for (int q = 0; q < CT; ++q) dred[w * Kp + lane * CT + q] = dO[q];
     }
     __syncthreads();
+    const int rep = blockIdx.x % kRep;
     unsigned long long sink = 0;
     for (int k = tid; k < K; k += kBlock) {
       double t = 0.0;
       for (int ww = 0; ww < kBlock / kWave; ++ww) t += dred[ww * Kp + k];
-      sink += fixed_add_wait(&a.acc_out[(long long)seg * K + k], t, a.scale_o);
+      sink += fixed_add_wait(&a.acc_out[rep * a.epi.rep_stride + (long long)seg * K + k], t, a.scale_o);
     }
     km = block_sum(km, red);
     ent = block_sum(ent, red);
     if (tid == 0) {
-      sink += fixed_add_wait(&ctrl->obj_acc[0], km, a.scale_obj);
-      sink += fixed_add_wait(&ctrl->obj_acc[1], ent, a.scale_obj);
+      sink += fixed_add_wait(&a.epi.obj_acc[2 * rep], km, a.scale_obj);
+      sink += fixed_add_wait(&a.epi.obj_acc[2 * rep + 1], ent, a.scale_obj);
     }
     asm volatile("" ::"v"(sink));
   }
@@ -641,8 +661,6 @@ __global__ void ctrl_reset_kernel(Ctrl* c) {
     c->done = 0;
     c->n_obj = 0;
     for (int i = 0; i < 4; ++i) c->hist[i] = 0.0;
-    c->obj_acc[0] = 0;
-    c->obj_acc[1] = 0;
   }
 }
 

@@ -1,0 +1,537 @@
+// Matrix-core (MFMA) kernels for FP32 storage on gfx950.
+//
+// Two GEMM-shaped contractions run every clustering iteration:
+//   centroid pass   Y   = R^T Z_norm            (K x d, reduced over N rows)
+//   row kernel      sim = Z_tile Y_norm^T       (16 rows x K per MFMA tile)
+// (reference: the cuBLAS GEMMs of clustering.cu:272-289).
+//
+// Precision modes:
+//   kPrecF32   v_mfma_f32_16x16x4f32: FP32 products, FP32 accumulation.
+//   kPrecBF16  v_mfma_f32_16x16x32_bf16 on a hi/lo split of each FP32 operand
+//              (x = hi + lo, hi = bf16(x), lo = bf16(x - hi)); the product is
+//              hi*hi' + hi*lo' + lo*hi' with FP32 accumulation.
+//
+// Lane layouts (checked on gfx950): for the 16x16x4 f32 op lane l holds
+// A[l%16][l/16], B[l/16][l%16] and D[4*(l/16)+r][l%16] (r = 0..3); for the
+// 16x16x32 bf16 op lane l holds A[l%16][8*(l/16)+v], B[8*(l/16)+v][l%16]
+// (v = 0..7) and the same D.
+#pragma once
+
+#include "hz_cluster.hpp"
+
+namespace hz {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int kPrecF32 = 0;
+constexpr int kPrecBF16 = 1;
+
+__device__ __forceinline__ floatx4 mfma_f32(float a, float b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ floatx4 mfma_bf16(bf16x8 a, bf16x8 b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// hi/lo split of 8 floats (round to nearest even on both parts).
+__device__ __forceinline__ void split_bf16(const float (&x)[8], bf16x8& hi, bf16x8& lo) {
+#pragma unroll
+  for (int v = 0; v < 8; ++v) {
+    __bf16 h = (__bf16)x[v];
+    hi[v] = h;
+    lo[v] = (__bf16)(x[v] - (float)h);
+  }
+}
+
+// ---------------------------------------------------------------- centroid prep
+
+// Normalised centroids in the two fragment layouts the row kernel reads:
+//   YT[c * Kp + k]           FP32, c-major (B operand of the f32 op)
+//   YH/YL[k * dP + c]        BF16 hi/lo, k-major, dP = round_up(d, 32)
+// One workgroup per centroid row (its L2 norm is computed in `yreduce`).
+__global__ void centroid_split_kernel(const float* __restrict__ YT, __bf16* __restrict__ YH,
+                                      __bf16* __restrict__ YL, int K, int Kp, int d, int dP,
+                                      const Ctrl* ctrl) {
+  if (ctrl && ctrl->done) return;
+  const int k = blockIdx.x;
+  for (int c = threadIdx.x; c < dP; c += blockDim.x) {
+    float x = (k < K && c < d) ? YT[(long long)c * Kp + k] : 0.f;
+    __bf16 h = (__bf16)x;
+    YH[(long long)k * dP + c] = h;
+    YL[(long long)k * dP + c] = (__bf16)(x - (float)h);
+  }
+}
+
+// ---------------------------------------------------------------- row kernel
+
+struct RowMArgs {
+  const float* Z;
+  const float* YT;    // d x Kp (f32 mode)
+  const __bf16* YH;   // Kp x dP (bf16 mode)
+  const __bf16* YL;
+  float* R;
+  const float* pen;   // B x K, nullptr = unit penalty
+  const int* order;
+  const int* seg_ch;
+  const int* ch_start;
+  const int* ch_len;
+  const int* ch_seg;
+  long long* acc_out;
+  int* counter;
+  int seg_base, block;
+  float term;
+  int n, d, K, Kp, dP;
+  int smem_bytes;
+  double scale_o, scale_obj;
+  float* obj_out;
+  EpiArgs<float> epi;
+};
+
+// Each wave owns one chunk (rows of one batch in one block) and walks it in
+// 16-row MFMA tiles; the workgroup's four waves share the LDS copy of the
+// normalised centroids.  Epilogue per row as in kernels_pen.cuh:27-82:
+// v = exp(term (1 - sim)) * pen[batch], R_out = v / sum(v).
+// LDS bytes of the staged centroids: f32 = 4*SM zero-padded rows of YT;
+// bf16 = YH and YL with dP = 32*SM columns.
+template <int PREC, int SM>
+__host__ __device__ inline size_t row_mfma_lds(int Kp) {
+  return (PREC == kPrecF32) ? (size_t)4 * SM * (Kp + 16) * 4 : (size_t)Kp * (32 * SM + 8) * 2 * 2;
+}
+
+// SM = number of MFMA k-steps over the d columns (4 per step for f32, 32 for
+// bf16), a compile-time constant so the matrix loop is straight-line code.
+template <int PREC, int NT, int SM>
+__global__ __launch_bounds__(kBlock) void row_mfma_kernel(RowMArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ int s_last;
+  __shared__ double red[kBlock / kWave];
+  __shared__ float s_part[kBlock / kWave][16 * NT];
+  __shared__ int s_seg[kBlock / kWave];
+  __shared__ double s_obj2[kBlock / kWave][2];
+  Ctrl* ctrl = a.epi.ctrl;
+  if (ctrl->done) return;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
+  const int lr = lane & 15, lg = lane >> 4;  // row/col inside a 16-wide fragment, k group
+  const int d = a.d, K = a.K, Kp = a.Kp, B = a.epi.B;
+  const int nch_lo = a.seg_ch[a.seg_base];
+  const int nch_hi = a.seg_ch[a.seg_base + B];
+  const int c0 = nch_lo + blockIdx.x * (kBlock / kWave);
+  const bool any = c0 < nch_hi;
+
+  // this wave's chunk; its cell ids and the first tile's A operand are
+  // requested before the centroid staging so the loads overlap it
+  const int c = c0 + wv;
+  const bool has = c < nch_hi;
+  int seg = 0, p0 = 0, len = 0, my_cell = -1;
+  if (has) {
+    seg = a.ch_seg[c];
+    p0 = a.ch_start[c];
+    len = a.ch_len[c];
+    my_cell = (lane < len) ? a.order[p0 + lane] : -1;
+  }
+  constexpr int NA = (PREC == kPrecF32) ? SM : 8 * SM;  // A values per lane per tile
+  float za[NA];
+  auto load_a = [&](int r0) {
+    const int zc = __shfl(my_cell, (r0 + lr) & 63, kWave);
+    const bool zvalid = has && (r0 + lr < len);
+    // loads use a valid address (cell 0 / column d-1) and are masked after;
+    // 32-bit element offsets (the host keeps N*d and N*K below 2^31)
+    const unsigned zbase = (unsigned)(zvalid ? zc : 0) * (unsigned)d;
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+      const int cc = (PREC == kPrecF32) ? 4 * i + lg : 32 * (i / 8) + 8 * lg + (i % 8);
+      const float zv = a.Z[zbase + (unsigned)min(cc, d - 1)];
+      za[i] = (zvalid && cc < d) ? zv : 0.f;
+    }
+  };
+  load_a(0);
+
+  // stage the normalised centroids (row strides padded against bank conflicts)
+  const int KpS = Kp + 16;      // f32: floats per LDS row of YT
+  const int dPS = a.dP + 8;     // bf16: elements per LDS row of YH / YL
+  if (any) {
+    if (PREC == kPrecF32) {
+      // rows d .. 4*SM-1 are zero so the k-loop needs no bounds test
+      const float4* src = reinterpret_cast<const float4*>(a.YT);
+      float4* dst = reinterpret_cast<float4*>(smem);
+      const int per = Kp / 4, perS = KpS / 4;
+      for (int e = tid; e < 4 * SM * per; e += kBlock) {
+        const int row = e / per;
+        dst[row * perS + e % per] = (row < d) ? src[e] : float4{0.f, 0.f, 0.f, 0.f};
+      }
+    } else {
+      const uint4* sh = reinterpret_cast<const uint4*>(a.YH);
+      const uint4* sl = reinterpret_cast<const uint4*>(a.YL);
+      uint4* dst = reinterpret_cast<uint4*>(smem);
+      const int per = a.dP / 8, perS = dPS / 8;
+      const int half = Kp * perS;
+      for (int e = tid; e < Kp * per; e += kBlock) {
+        const int o = (e / per) * perS + e % per;
+        dst[o] = sh[e];
+        dst[half + o] = sl[e];
+      }
+    }
+  }
+  __syncthreads();
+
+  if (has) {
+    const int b = seg - a.seg_base;
+    float pv[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const int k = 16 * t + lr;
+      pv[t] = (k < K) ? (a.pen ? a.pen[(long long)b * K + k] : 1.f) : 0.f;
+    }
+    float dO[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) dO[t] = 0.f;
+    float kmf = 0.f, entf = 0.f;
+    const float* YTs = reinterpret_cast<const float*>(smem);
+    const __bf16* YHs = reinterpret_cast<const __bf16*>(smem);
+    const __bf16* YLs = YHs + (size_t)Kp * dPS;
+    for (int r0 = 0; r0 < len; r0 += 16) {
+      if (r0 > 0) load_a(r0);  // the first tile was requested before staging
+      floatx4 acc[NT];
+#pragma unroll
+      for (int t = 0; t < NT; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+      if (PREC == kPrecF32) {
+#pragma unroll
+        for (int s = 0; s < SM; ++s) {
+          const int cc = 4 * s + lg;
+#pragma unroll
+          for (int t = 0; t < NT; ++t) acc[t] = mfma_f32(za[s], YTs[cc * KpS + 16 * t + lr], acc[t]);
+        }
+      } else {
+#pragma unroll
+        for (int s = 0; s < SM; ++s) {
+          float xs[8];
+#pragma unroll
+          for (int v = 0; v < 8; ++v) xs[v] = za[8 * s + v];
+          bf16x8 ah, al;
+          split_bf16(xs, ah, al);
+#pragma unroll
+          for (int t = 0; t < NT; ++t) {
+            const int off = (16 * t + lr) * dPS + 32 * s + 8 * lg;
+            const bf16x8 bh = *reinterpret_cast<const bf16x8*>(YHs + off);
+            const bf16x8 bl = *reinterpret_cast<const bf16x8*>(YLs + off);
+            acc[t] = mfma_bf16(ah, bh, acc[t]);
+            acc[t] = mfma_bf16(ah, bl, acc[t]);
+            acc[t] = mfma_bf16(al, bh, acc[t]);
+          }
+        }
+      }
+      // epilogue: lane holds rows 4*lg + r, column 16 t + lr
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int rr = r0 + 4 * lg + r;
+        const int rcell = __shfl(my_cell, rr & 63, kWave);
+        float v[NT];
+        float s = 0.f;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          v[t] = (16 * t + lr < K) ? dexp(a.term * (1.f - acc[t][r])) * pv[t] : 0.f;
+          s += v[t];
+        }
+        s = group_sum(s, 16);
+        const float inv = 1.f / s;
+        float s2 = 0.f;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          v[t] *= inv;
+          s2 += v[t];
+        }
+        s2 = group_sum(s2, 16);
+        const float inv2 = 1.f / s2;
+        if (rr < len) {
+          float* dst = a.R + (unsigned)rcell * (unsigned)K;
+#pragma unroll
+          for (int t = 0; t < NT; ++t) {
+            const int k = 16 * t + lr;
+            if (k < K) {
+              dst[k] = v[t];
+              kmf += v[t] * 2.f * (1.f - acc[t][r]);
+              const float xx = v[t] * inv2;
+              entf += xx * __logf(xx + 1e-12f);  // objective term only
+              dO[t] += v[t];
+            }
+          }
+        }
+      }
+    }
+    // chunk contribution to O: reduce the four row groups of the wave
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      float x = dO[t];
+      x += __shfl_xor(x, 16, kWave);
+      x += __shfl_xor(x, 32, kWave);
+      const int k = 16 * t + lr;
+      if (lg == 0 && k < K) s_part[wv][k] = x;
+    }
+    const double km = wave_sum((double)kmf);
+    const double ent = wave_sum((double)entf);
+    if (lane == 0) {
+      s_seg[wv] = seg;
+      s_obj2[wv][0] = km;
+      s_obj2[wv][1] = ent;
+    }
+  } else if (lane == 0) {
+    s_seg[wv] = -1;
+    s_obj2[wv][0] = 0.0;
+    s_obj2[wv][1] = 0.0;
+  }
+  __syncthreads();
+  // workgroup-level combine: consecutive waves of one segment share one add;
+  // all adds are issued before any result is consumed
+  {
+    const int rep = blockIdx.x % kRep;
+    long long* acc = a.acc_out + rep * a.epi.rep_stride;
+    unsigned long long sink = 0;
+    for (int k = tid; k < K; k += kBlock) {
+      int w = 0;
+      while (w < kBlock / kWave) {
+        const int sg = s_seg[w];
+        if (sg < 0) {
+          ++w;
+          continue;
+        }
+        float x = 0.f;
+        int w2 = w;
+        while (w2 < kBlock / kWave && s_seg[w2] == sg) x += s_part[w2++][k];
+        sink += fixed_add_wait(&acc[(long long)sg * K + k], (double)x, a.scale_o);
+        w = w2;
+      }
+    }
+    if (tid == 0) {
+      double km4 = 0.0, ent4 = 0.0;
+      for (int w = 0; w < kBlock / kWave; ++w) {
+        km4 += s_obj2[w][0];
+        ent4 += s_obj2[w][1];
+      }
+      const unsigned long long r0 = fixed_add_wait(&a.epi.obj_acc[2 * rep], km4, a.scale_obj);
+      const unsigned long long r1 = fixed_add_wait(&a.epi.obj_acc[2 * rep + 1], ent4, a.scale_obj);
+      sink += r0 + r1;
+    }
+    asm volatile("" ::"v"(sink));
+  }
+  if (!last_to_arrive(a.counter, &s_last)) return;
+  const EpiArgs<float>& e = a.epi;
+  if (a.block < 0) {
+    float obj = block_update(e, smem, a.smem_bytes, 0, 0, kUpdOverwrite | kUpdObjInit, red);
+    if (tid == 0 && a.obj_out) *a.obj_out = obj;
+  } else if (a.block + 1 < e.nblk) {
+    block_update(e, smem, a.smem_bytes, a.seg_base, a.seg_base + B, kUpdOut | kUpdIn | kUpdPen,
+                 red);
+  } else {
+    block_update(e, smem, a.smem_bytes, a.seg_base, 0, kUpdOut | kUpdObj, red);
+  }
+  reset_counter(a.counter);
+}
+
+// ---------------------------------------------------------------- centroid pass
+
+struct CentMArgs {
+  const float* Z;
+  const float* R;
+  const int* order;
+  const int* seg_start;
+  const int* seg_ch;
+  const int* ch_start;
+  const int* ch_len;
+  const int* ch_seg;
+  long long* acc_in;
+  float* Ypart;  // G x K x d (one partial per workgroup)
+  const Ctrl* ctrl;
+  int nseg, cpw, n, d, K;
+  double scale_o;
+};
+
+// LDS row stride (elements) congruent to 16 mod 64 so the four row groups of
+// a fragment read hit disjoint banks.
+__host__ __device__ constexpr int lds_stride(int w) { return (w + 63) / 64 * 64 + 16; }
+
+template <int PREC, int NT, int NU>
+__host__ __device__ constexpr size_t centroid_tile_lds() {
+  // f32: R tile [64][NT*16] and Z tile [64][NU*16] (row-major, padded);
+  // bf16: transposed hi/lo tiles [NT*16][64] and [NU*16][64] (padded).
+  return PREC == kPrecF32 ? (size_t)64 * (lds_stride(16 * NT) + lds_stride(16 * NU)) * 4
+                          : (size_t)(16 * NT + 16 * NU) * lds_stride(64) * 2 * 2;
+}
+
+// Y_partial = R^T [Z | 1] over a contiguous range of chunks, one workgroup
+// partial; the column of ones yields each (block, batch) segment's column sums
+// of R, flushed to acc_in when the segment ends.  A segment's positions in
+// `order` are contiguous, so the workgroup walks each segment in 64-row tiles
+// that never straddle a batch boundary.  Tiles are loaded cooperatively
+// (coalesced along rows, the next tile prefetched into registers) and staged
+// in LDS; each wave owns a quarter of the NT x NU output tiles.
+template <int PREC, int NT, int NU>
+__global__ __launch_bounds__(kBlock) void centroid_mfma_kernel(CentMArgs a) {
+  constexpr int TR = 64;
+  constexpr int NTILE = NT * NU;
+  constexpr int MT = (NTILE + 3) / 4;  // output tiles per wave
+  constexpr int LWR = (16 * NT <= 64) ? 64 : (16 * NT <= 128 ? 128 : 256);  // R load width
+  constexpr int LWZ = (16 * NU <= 32) ? 32 : (16 * NU <= 64 ? 64 : 128);    // Z load width
+  constexpr int MR = TR * LWR / kBlock;  // R elements per thread per tile
+  constexpr int MZ = TR * LWZ / kBlock;
+  constexpr int SR = lds_stride(16 * NT), SZ = lds_stride(16 * NU), ST = lds_stride(TR);
+  extern __shared__ __align__(16) unsigned char smem[];
+  if (a.ctrl->done) return;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
+  const int lr = lane & 15, lg = lane >> 4;
+  const int d = a.d, K = a.K;
+  const int nch = a.seg_ch[a.nseg];
+  const int c0 = blockIdx.x * a.cpw;
+  const int c1 = min(nch, c0 + a.cpw);
+  const int ud = d >> 4, jd = d & 15;  // tile / column of the ones column
+
+  float* Rs = reinterpret_cast<float*>(smem);
+  float* Zs = Rs + TR * SR;
+  __bf16* RTh = reinterpret_cast<__bf16*>(smem);
+  __bf16* RTl = RTh + 16 * NT * ST;
+  __bf16* ZTh = RTl + 16 * NT * ST;
+  __bf16* ZTl = ZTh + 16 * NU * ST;
+
+  floatx4 acc[MT];
+#pragma unroll
+  for (int j = 0; j < MT; ++j) acc[j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  __shared__ int s_cell[TR];
+  // load rows [p, p + rows) of `order` into the LDS tiles (zero beyond rows / K / d+1)
+  auto load_tile = [&](int p, int rows) {
+    if (tid < TR) s_cell[tid] = a.order[p + min(tid, rows - 1)];
+    __syncthreads();
+    // 32-bit element offsets (the host keeps N*K and N*d below 2^31) let the
+    // loads use a scalar base plus a 32-bit vector offset
+#pragma unroll 8
+    for (int m = 0; m < MR; ++m) {
+      const int e = tid + m * kBlock;
+      const int r = e / LWR, k = e % LWR;
+      if (k < 16 * NT) {
+        const float x = a.R[(unsigned)s_cell[r] * (unsigned)K + (unsigned)min(k, K - 1)];
+        const float v = (r < rows && k < K) ? x : 0.f;
+        if (PREC == kPrecF32) {
+          Rs[r * SR + k] = v;
+        } else {
+          const __bf16 h = (__bf16)v;
+          RTh[k * ST + r] = h;
+          RTl[k * ST + r] = (__bf16)(v - (float)h);
+        }
+      }
+    }
+#pragma unroll 8
+    for (int m = 0; m < MZ; ++m) {
+      const int e = tid + m * kBlock;
+      const int r = e / LWZ, cc = e % LWZ;
+      if (cc < 16 * NU) {
+        const float x = a.Z[(unsigned)s_cell[r] * (unsigned)d + (unsigned)min(cc, d - 1)];
+        const float v = (r < rows) ? (cc < d ? x : (cc == d ? 1.f : 0.f)) : 0.f;
+        if (PREC == kPrecF32) {
+          Zs[r * SZ + cc] = v;
+        } else {
+          const __bf16 h = (__bf16)v;
+          ZTh[cc * ST + r] = h;
+          ZTl[cc * ST + r] = (__bf16)(v - (float)h);
+        }
+      }
+    }
+  };
+
+  // walk the segments of the chunk range; a segment's positions are
+  // contiguous, so its tiles are [p, min(seg_start[seg+1], range end))
+  const int range_end = (c1 > c0) ? a.ch_start[c1 - 1] + a.ch_len[c1 - 1] : 0;
+  int c = c0;
+  int seg = 0, p = 0, pend = 0;
+  if (c < c1) {
+    seg = a.ch_seg[c];
+    p = a.ch_start[c];
+    pend = min(a.seg_start[seg + 1], range_end);
+  }
+  while (c < c1) {
+    const int rows = min(TR, pend - p);
+    __syncthreads();  // previous tile's LDS reads are done
+    load_tile(p, rows);
+    __syncthreads();
+    const bool seg_done = p + rows >= pend;
+    // MFMA over the staged tile
+    if (PREC == kPrecF32) {
+#pragma unroll 4
+      for (int q = 0; q < TR / 4; ++q) {
+#pragma unroll
+        for (int j = 0; j < MT; ++j) {
+          const int i = wv + 4 * j;
+          if (i < NTILE) {
+            const int t = i % NT, u = i / NT;
+            const float av = Rs[(4 * q + lg) * SR + 16 * t + lr];
+            const float bv = Zs[(4 * q + lg) * SZ + 16 * u + lr];
+            acc[j] = mfma_f32(av, bv, acc[j]);
+          }
+        }
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < TR / 32; ++q) {
+#pragma unroll
+        for (int j = 0; j < MT; ++j) {
+          const int i = wv + 4 * j;
+          if (i < NTILE) {
+            const int t = i % NT, u = i / NT;
+            const int ra = (16 * t + lr) * ST + 32 * q + 8 * lg;
+            const int rb = (16 * u + lr) * ST + 32 * q + 8 * lg;
+            const bf16x8 ah = *reinterpret_cast<const bf16x8*>(RTh + ra);
+            const bf16x8 al = *reinterpret_cast<const bf16x8*>(RTl + ra);
+            const bf16x8 bh = *reinterpret_cast<const bf16x8*>(ZTh + rb);
+            const bf16x8 bl = *reinterpret_cast<const bf16x8*>(ZTl + rb);
+            acc[j] = mfma_bf16(ah, bh, acc[j]);
+            acc[j] = mfma_bf16(ah, bl, acc[j]);
+            acc[j] = mfma_bf16(al, bh, acc[j]);
+          }
+        }
+      }
+    }
+    if (seg_done) {
+      // flush the ones column (tile u == ud, column jd) into acc_in
+#pragma unroll
+      for (int j = 0; j < MT; ++j) {
+        const int i = wv + 4 * j;
+        if (i < NTILE && i / NT == ud && lr == jd) {
+          const int t = i % NT;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int k = 16 * t + 4 * lg + r;
+            if (k < K) fixed_add(&a.acc_in[(long long)seg * K + k], (double)acc[j][r], a.scale_o);
+          }
+          acc[j] = floatx4{0.f, 0.f, 0.f, 0.f};
+        }
+      }
+    }
+    if (seg_done) {
+      c = a.seg_ch[seg + 1];  // first chunk after this segment
+      if (c < c1) {
+        seg = a.ch_seg[c];
+        p = a.ch_start[c];
+        pend = min(a.seg_start[seg + 1], range_end);
+      }
+    } else {
+      p += rows;
+    }
+  }
+  // write this workgroup's K x d partial
+  float* out = a.Ypart + (long long)blockIdx.x * K * d;
+#pragma unroll
+  for (int j = 0; j < MT; ++j) {
+    const int i = wv + 4 * j;
+    if (i < NTILE) {
+      const int t = i % NT, u = i / NT;
+      const int cc = 16 * u + lr;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int k = 16 * t + 4 * lg + r;
+        if (k < K && cc < d) out[(long long)k * d + cc] = acc[j][r];
+      }
+    }
+  }
+}
+
+}  // namespace hz
