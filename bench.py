@@ -29,6 +29,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "Harmony cells·iters/s (split-BF16 vs FP32, ≤1e-3 rel err) + p50/p99 job latency"
 HBM_PEAK_GBS = 8000.0
+COPY_BYTES = 256 << 20
 
 WORKLOADS = {
     # name: (n_cells, n_pcs, n_types, n_batches, n_clusters)
@@ -92,6 +93,7 @@ def roofline(job, first_iter: int, steps: int, esz: int) -> tuple[dict, dict]:
     # algorithmic bytes per launch (reads and writes each array once)
     bytes_per = {
         "row": esz * (n / nblk) * (K + d),          # read Z rows, write R rows
+        "sweep": esz * n * (K + d),                 # all blocks: read Z rows, write R rows
         "centroid": esz * n * (K + d),              # read R and Z
         "phi": esz * n * (K + d),                   # read R and X
         "apply": esz * n * (K + 3 * d),             # read R, X; write Z_hat, Z_norm
@@ -120,7 +122,8 @@ def roofline(job, first_iter: int, steps: int, esz: int) -> tuple[dict, dict]:
             "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic}
     kernels["_per_clustering_iter_ms"] = round(
         sum(v["total_ms"] for k, v in kernels.items()
-            if k in ("row", "centroid", "yreduce", "order")) / max(n_clust, 1), 4)
+            if k in ("row", "sweep", "centroid", "yreduce")) / max(n_clust, 1), 4)
+    # the block order is built on a side stream, overlapped with the iteration before
     return roof, kernels
 
 
@@ -166,6 +169,11 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=50_000)
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--pmc-run", action="store_true",
+                    help="short run for hardware-counter passes: a 256 MiB device copy "
+                         "(calibration kernel of known size) then 1 Harmony iteration")
+    ap.add_argument("--tune", action="append", default=[],
+                    help="library tuning knob key=value (row_chunk, centroid_wg, no_sweep, sweep_wg)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the Harmony path "
@@ -177,6 +185,9 @@ def main() -> int:
     from rapids_singlecell_amd import _native
 
     _native.load()
+    tune = {k: int(v) for k, v in (t.split("=", 1) for t in args.tune)}
+    if tune:
+        _native.set_tuning(**tune)
     prec = {"native": 0, "split_bf16": 1}[args.precision]
     X, codes, B, K = make_data(args.workload)
     n, d = X.shape
@@ -185,6 +196,18 @@ def main() -> int:
     job = build_job(X, codes, B, K, prec)
     job.init()
     torch.cuda.synchronize()
+    if args.pmc_run:
+        # calibration: a plain copy reads and writes exactly COPY_BYTES
+        src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        job.step(0)
+        torch.cuda.synchronize()
+        print(json.dumps({"pmc_run": True, "copy_bytes": COPY_BYTES, "workload": args.workload,
+                          "precision": args.precision}), flush=True)
+        return 0
     it = 0
     for _ in range(args.warmup):
         job.step(it)
@@ -220,6 +243,7 @@ def main() -> int:
             "workload": f"harmony1 N={n} d={d} K={K} B={B} theta=2 sigma=0.1 lambda=1 "
                         f"block_proportion=0.05, 20 clustering iters per Harmony iter",
             "precision": args.precision,
+            "tuning": _native.get_tuning(),
             "n_cells": n, "n_pcs": d, "n_clusters": K, "n_batches": B,
         },
         "roofline": roof,

@@ -71,6 +71,7 @@ const char* hz_last_error(void);
 #define HZ_PROF_CORRECTION 4 /* whole correction */
 #define HZ_PROF_PHI 5        /* correction: Phi pass */
 #define HZ_PROF_APPLY 6      /* correction: apply + renormalise */
+#define HZ_PROF_SWEEP 7      /* all blocks of one clustering iteration, one launch */
 #define HZ_PROF_KINDS 8
 int hz_profile_enable(int on);
 int hz_profile_collect(double* ms_out, long long* launches_out, int n_kinds);
@@ -81,6 +82,8 @@ int hz_profile_collect(double* ms_out, long long* launches_out, int n_kinds);
  * closest analogue is the colsum_algo benchmark, _harmony/_helper.py:401-504). */
 #define HZ_TUNE_ROW_CHUNK 0   /* rows per chunk of the row kernel (multiple of 16) */
 #define HZ_TUNE_CENTROID_WG 1 /* workgroups of the centroid pass */
+#define HZ_TUNE_SWEEP 2       /* 0 = block sweep in one launch when it fits, 1 = one launch per block */
+#define HZ_TUNE_SWEEP_WG 3    /* workgroups of the block sweep (capped at what is co-resident) */
 #define HZ_TUNE_KEYS 4
 int hz_set_tuning(int key, int value);
 int hz_get_tuning(int key);

@@ -8,6 +8,7 @@
 
 #include "hz_correct.hpp"
 #include "hz_mfma.hpp"
+#include "hz_sweep.hpp"
 
 namespace hz {
 
@@ -158,6 +159,7 @@ struct Layout {
   // clustering
   PermBufs pb;
   ChunkTab ct;
+  ChunkTab ct2;  // second block-order table: iteration it+1 is built while it runs
   int* nch_tmp;
   ChunkTab st;
   int* nch_tmp_s;
@@ -168,7 +170,7 @@ struct Layout {
   long long* obj_acc;  // kRep x 2
   Ctrl* ctrl;
   Ctrl* ctrl_static;  // never set done: used by the correction pass
-  int* counters;      // nblk + 4
+  int* counters;      // nblk + 6
   void* Ypart;
   void* Y;
   void* YT;
@@ -228,7 +230,7 @@ static int make_layout(const hz_dims* dm, void* base, Layout& L) {
   if (g_tune[HZ_TUNE_ROW_CHUNK] > 0)
     L.chunk = g_tune[HZ_TUNE_ROW_CHUNK];
   else if (L.mfma)
-    L.chunk = (L.bs <= 32768) ? 16 : 32;
+    L.chunk = 16;  // one MFMA tile per chunk
   else
     L.chunk = (L.bs <= 32768) ? 64 : 128;
   if (L.mfma) L.chunk = std::min(L.chunk, 64);  // a wave holds a chunk's cell ids
@@ -259,6 +261,7 @@ static int make_layout(const hz_dims* dm, void* base, Layout& L) {
 
   Carve c(base);
   carve_perm(c, L.pd, L.pb, L.ct);
+  carve_tab(c, L.pd, L.ct2);
   L.nch_tmp = c.take<int>(nseg + 1);
   {
     PermBufs dummy;
@@ -283,7 +286,7 @@ static int make_layout(const hz_dims* dm, void* base, Layout& L) {
   L.obj_acc = c.take<long long>(2 * kRep);
   L.ctrl = c.take<Ctrl>(1);
   L.ctrl_static = c.take<Ctrl>(1);
-  L.counters = c.take<int>(L.pd.nblk + 4);
+  L.counters = c.take<int>(L.pd.nblk + 6);
   L.Ypart = c.take<char>((size_t)L.G * KD * L.esz);
   L.Y = c.take<char>((size_t)KD * L.esz);
   L.YT = c.take<char>((size_t)L.d * L.Kp * L.esz);
@@ -451,6 +454,125 @@ static int launch_row_mfma(const Layout& L, const RowMArgs& a, int grid, hipStre
   return launch_row_mfma_p<kPrecF32>(L, a, grid, s);
 }
 
+// ---------------------------------------------------------------- block sweep
+
+static bool g_sweep_broken[64];  // cooperative launch refused on this device
+
+// Returns HZ_OK after a launch, HZ_ERR_UNSUPPORTED (nothing launched) when the
+// grid cannot be made co-resident; other codes are errors.
+template <int PREC, int NT, int SM>
+static int launch_sw(const Layout& L, const SweepArgs& a, hipStream_t s) {
+  auto kern = sweep_kernel<PREC, NT, SM>;
+  const size_t lds = sweep_lds<PREC, SM>(L.Kp, L.B, L.K);
+  if (lds > (size_t)device_lds_limit()) return HZ_ERR_UNSUPPORTED;
+  if (lds > 65536)
+    HZ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  int per_cu = 0;
+  HZ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern),
+                                                      kBlock, lds));
+  if (per_cu < 1) return HZ_ERR_UNSUPPORTED;
+  const long long cap = (long long)per_cu * device_cu_count();
+  long long G = g_tune[HZ_TUNE_SWEEP_WG] > 0 ? g_tune[HZ_TUNE_SWEEP_WG] : device_cu_count();
+  G = std::max<long long>(1, std::min(G, cap));
+  void* args[] = {const_cast<SweepArgs*>(&a)};
+  hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kern), dim3((unsigned)G),
+                                            dim3(kBlock), args, (unsigned)lds, s);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    g_sweep_broken[current_device()] = true;
+    return HZ_ERR_UNSUPPORTED;
+  }
+  return HZ_OK;
+}
+
+template <int PREC, int SM>
+static int launch_sw_sm(const Layout& L, const SweepArgs& a, hipStream_t s) {
+  switch (L.NT) {
+    case 4: return launch_sw<PREC, 4, SM>(L, a, s);
+    case 7: return launch_sw<PREC, 7, SM>(L, a, s);
+    case 8: return launch_sw<PREC, 8, SM>(L, a, s);
+    default: return launch_sw<PREC, 16, SM>(L, a, s);
+  }
+}
+
+static int launch_insum(const Layout& L, const InSumArgs& a, hipStream_t s) {
+  const int grid = (int)std::max<long long>(1, ceil_div(L.pd.maxch, kBlock / kWave));
+  switch ((int)ceil_div(L.K, kWave)) {
+    case 1: hipLaunchKernelGGL(insum_kernel<1>, dim3(grid), dim3(kBlock), 0, s, a); break;
+    case 2: hipLaunchKernelGGL(insum_kernel<2>, dim3(grid), dim3(kBlock), 0, s, a); break;
+    case 3: hipLaunchKernelGGL(insum_kernel<3>, dim3(grid), dim3(kBlock), 0, s, a); break;
+    case 4: hipLaunchKernelGGL(insum_kernel<4>, dim3(grid), dim3(kBlock), 0, s, a); break;
+    default: set_error("old-row sums support at most 256 clusters"); return HZ_ERR_UNSUPPORTED;
+  }
+  HZ_LAUNCHED("insum_kernel");
+  return HZ_OK;
+}
+
+static int launch_sweep(const Layout& L, const SweepArgs& a, hipStream_t s) {
+  if (L.prec == HZ_PREC_SPLIT_BF16) {
+    switch (row_sm(L)) {
+      case 1: return launch_sw_sm<kPrecBF16, 1>(L, a, s);
+      case 2: return launch_sw_sm<kPrecBF16, 2>(L, a, s);
+      case 3: return launch_sw_sm<kPrecBF16, 3>(L, a, s);
+      default: return launch_sw_sm<kPrecBF16, 4>(L, a, s);
+    }
+  }
+  switch (row_sm(L)) {
+    case 4: return launch_sw_sm<kPrecF32, 4>(L, a, s);
+    case 8: return launch_sw_sm<kPrecF32, 8>(L, a, s);
+    case 13: return launch_sw_sm<kPrecF32, 13>(L, a, s);
+    case 16: return launch_sw_sm<kPrecF32, 16>(L, a, s);
+    case 25: return launch_sw_sm<kPrecF32, 25>(L, a, s);
+    default: return launch_sw_sm<kPrecF32, 32>(L, a, s);
+  }
+}
+
+template <int PREC, int NT, int NU>
+static int launch_cs(const CentSArgs& a, int grid, hipStream_t s) {
+  if constexpr (NT * NU <= 32) {
+    auto kern = centroid_stream_kernel<PREC, NT, NU>;
+    const size_t lds = centroid_tile_lds<PREC, NT, NU>();
+    HZ_RET_IF(lds > (size_t)device_lds_limit(), HZ_ERR_UNSUPPORTED,
+              "centroid pass needs %zu bytes of LDS", lds);
+    if (lds > 65536)
+      HZ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), lds, s, a);
+    HZ_LAUNCHED("centroid_stream_kernel");
+    return HZ_OK;
+  } else {
+    set_error("centroid tile shape %d x %d not instantiated", NT, NU);
+    return HZ_ERR_UNSUPPORTED;
+  }
+}
+
+template <int PREC, int NT>
+static int launch_cs_nt(const Layout& L, const CentSArgs& a, int grid, hipStream_t s) {
+  switch (L.NU) {
+    case 2: return launch_cs<PREC, NT, 2>(a, grid, s);
+    case 3: return launch_cs<PREC, NT, 3>(a, grid, s);
+    case 4: return launch_cs<PREC, NT, 4>(a, grid, s);
+    case 7: return launch_cs<PREC, NT, 7>(a, grid, s);
+    default: return launch_cs<PREC, NT, 8>(a, grid, s);
+  }
+}
+
+template <int PREC>
+static int launch_cs_p(const Layout& L, const CentSArgs& a, int grid, hipStream_t s) {
+  switch (L.NT) {
+    case 4: return launch_cs_nt<PREC, 4>(L, a, grid, s);
+    case 7: return launch_cs_nt<PREC, 7>(L, a, grid, s);
+    case 8: return launch_cs_nt<PREC, 8>(L, a, grid, s);
+    default: return launch_cs_nt<PREC, 16>(L, a, grid, s);
+  }
+}
+
+static int launch_cent_stream(const Layout& L, const CentSArgs& a, int grid, hipStream_t s) {
+  if (L.prec == HZ_PREC_SPLIT_BF16) return launch_cs_p<kPrecBF16>(L, a, grid, s);
+  return launch_cs_p<kPrecF32>(L, a, grid, s);
+}
+
 template <int PREC, int NT, int NU>
 static int launch_cm(const CentMArgs& a, int grid, hipStream_t s) {
   if constexpr (NT * NU <= 32) {
@@ -537,6 +659,28 @@ static int read_done(const Ctrl* ctrl, hipStream_t s, int* done) {
   return HZ_OK;
 }
 
+// Per-thread side stream (default priority) that builds the next clustering
+// iteration's block order while the current one runs, with the events that
+// order it against the caller's stream.
+struct SideStream {
+  hipStream_t s = nullptr;
+  hipEvent_t start = nullptr, done = nullptr;
+  hipEvent_t ready[2] = {nullptr, nullptr};
+  hipEvent_t freed[2] = {nullptr, nullptr};
+};
+static thread_local SideStream t_side[64];
+
+static int side_stream(SideStream** out) {
+  SideStream& ss = t_side[current_device()];
+  if (!ss.s) {
+    HZ_HIP(hipStreamCreateWithFlags(&ss.s, hipStreamNonBlocking));
+    hipEvent_t* evs[] = {&ss.start, &ss.done, &ss.ready[0], &ss.ready[1], &ss.freed[0], &ss.freed[1]};
+    for (hipEvent_t* e : evs) HZ_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
+  }
+  *out = &ss;
+  return HZ_OK;
+}
+
 // ---------------------------------------------------------------- jobs
 
 template <typename T>
@@ -554,7 +698,7 @@ static int init_impl(const hz_dims* dm, const T* Z, const T* Y0, const int* cats
   HZ_LAUNCHED("ctrl_reset");
   HZ_TRY(fill<long long>(L.acc_out, (long long)kRep * L.pd.nblk * L.B * L.K, 0, s));
   HZ_TRY(fill<long long>(L.obj_acc, 2 * kRep, 0, s));
-  HZ_TRY(fill<int>(L.counters, L.pd.nblk + 4, 0, s));
+  HZ_TRY(fill<int>(L.counters, L.pd.nblk + 6, 0, s));
   hipLaunchKernelGGL(centroids_to_yt_kernel<T>, dim3(1), dim3(kBlock), 0, s, Y0,
                      reinterpret_cast<T*>(L.YT), L.K, L.d, L.Kp);
   HZ_LAUNCHED("centroids_to_yt");
@@ -622,7 +766,7 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
   hipLaunchKernelGGL(ctrl_reset_kernel, dim3(1), dim3(64), 0, s, L.ctrl);
   HZ_LAUNCHED("ctrl_reset");
   // acc_out / obj_acc are cleared by the centroid reduction of every iteration
-  HZ_TRY(fill<int>(L.counters, L.pd.nblk + 4, 0, s));
+  HZ_TRY(fill<int>(L.counters, L.pd.nblk + 6, 0, s));
   if (last_obj) HZ_TRY(fill<T>(last_obj, 1, T(0), s));
 
   const EpiArgs<T> epi = make_epi<T>(L, O, E, Pr, theta, sigma, tol, stabilized, last_obj);
@@ -652,6 +796,7 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
   ya.K = L.K;
   ya.d = L.d;
   ya.Kp = L.Kp;
+  ya.setup_block0 = 1;
   ya.epi = epi;
   const int ygrid = L.K;
 
@@ -711,7 +856,114 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
     cm.K = L.K;
     cm.scale_o = L.scale_o;
   }
-  if (use_cm) ya.G = L.Gm;
+  bool use_sw = false;
+  SweepArgs sw{};
+  CentSArgs cs{};
+  int gs = 1;
+  if constexpr (std::is_same<T, float>::value) {
+    // 16-byte loads of R and Z rows in the streaming centroid pass
+    const bool aligned = (reinterpret_cast<uintptr_t>(Z) % 16 == 0) &&
+                         (reinterpret_cast<uintptr_t>(R) % 16 == 0);
+    use_sw = use_rm && use_cm && aligned && L.K <= 256 && g_tune[HZ_TUNE_SWEEP] == 0 &&
+             !g_sweep_broken[current_device()];
+    sw.Z = Z;
+    sw.YT = reinterpret_cast<const float*>(L.YT);
+    sw.YH = reinterpret_cast<const __bf16*>(L.YH);
+    sw.YL = reinterpret_cast<const __bf16*>(L.YL);
+    sw.R = R;
+    sw.order = L.ct.order;
+    sw.seg_ch = L.ct.seg_ch;
+    sw.ch_start = L.ct.ch_start;
+    sw.ch_len = L.ct.ch_len;
+    sw.ch_seg = L.ct.ch_seg;
+    sw.acc_in = L.acc_in;
+    sw.acc_out = L.acc_out;
+    sw.obj_acc = L.obj_acc;
+    sw.rep_stride = epi.rep_stride;
+    sw.O = O;
+    sw.E = E;
+    sw.Pr = Pr;
+    sw.theta = theta;
+    sw.ctrl = L.ctrl;
+    sw.last_obj = last_obj;
+    sw.bar = L.counters + L.pd.nblk + 2;
+    sw.fin = L.counters + L.pd.nblk + 3;
+    sw.flag = L.counters + L.pd.nblk + 4;
+    sw.n = L.n;
+    sw.d = L.d;
+    sw.K = L.K;
+    sw.Kp = L.Kp;
+    sw.dP = L.dP;
+    sw.B = L.B;
+    sw.nblk = L.pd.nblk;
+    sw.term = ra.term;
+    sw.sigma = (float)sigma;
+    sw.tol = (float)tol;
+    sw.stabilized = stabilized;
+    sw.scale_o = L.scale_o;
+    sw.inv_scale_o = 1.0 / L.scale_o;
+    sw.scale_obj = L.scale_obj;
+    sw.inv_scale_obj = 1.0 / L.scale_obj;
+    sw.spin_limit = 1u << 20;
+    cs.Z = Z;
+    cs.R = R;
+    cs.Ypart = reinterpret_cast<float*>(L.Ypart);
+    cs.ctrl = L.ctrl;
+    cs.n = L.n;
+    cs.d = L.d;
+    cs.K = L.K;
+    gs = (int)std::max<long long>(1, std::min<long long>(L.Gm, ceil_div(L.n, 64)));
+  }
+
+  // block orders are built one iteration ahead on a side stream, into two
+  // alternating tables
+  ChunkTab tabs[2] = {L.ct, L.ct2};
+  auto bind_tab = [&](const ChunkTab& t) {
+    ca.order = t.order;
+    ca.seg_ch = t.seg_ch;
+    ca.ch_start = t.ch_start;
+    ca.ch_len = t.ch_len;
+    ca.ch_seg = t.ch_seg;
+    ra.order = t.order;
+    ra.seg_ch = t.seg_ch;
+    ra.ch_start = t.ch_start;
+    ra.ch_len = t.ch_len;
+    ra.ch_seg = t.ch_seg;
+    rm.order = t.order;
+    rm.seg_ch = t.seg_ch;
+    rm.ch_start = t.ch_start;
+    rm.ch_len = t.ch_len;
+    rm.ch_seg = t.ch_seg;
+    cm.order = t.order;
+    cm.seg_start = t.seg_start;
+    cm.seg_ch = t.seg_ch;
+    cm.ch_start = t.ch_start;
+    cm.ch_len = t.ch_len;
+    cm.ch_seg = t.ch_seg;
+    sw.order = t.order;
+    sw.seg_ch = t.seg_ch;
+    sw.ch_start = t.ch_start;
+    sw.ch_len = t.ch_len;
+    sw.ch_seg = t.ch_seg;
+  };
+  SideStream* ss = nullptr;
+  HZ_TRY(side_stream(&ss));
+  const hipStream_t s2 = ss->s;
+  InSumArgs ia{};
+  ia.R = reinterpret_cast<const float*>(R);
+  ia.acc_in = L.acc_in;
+  ia.nseg = (int)nseg;
+  ia.K = L.K;
+  ia.scale_o = L.scale_o;
+  auto enqueue_order = [&](int it) -> int {
+    ProfScope ps(HZ_PROF_ORDER, s2);
+    return launch_block_order(L.pd, seed + (uint32_t)it, cats, L.pb, tabs[it & 1], L.nch_tmp,
+                              nullptr, s2);
+  };
+  // the side stream starts after the work already queued on the caller's stream
+  HZ_HIP(hipEventRecord(ss->start, s));
+  HZ_HIP(hipStreamWaitEvent(s2, ss->start, 0));
+  HZ_TRY(enqueue_order(0));
 
   const int check_every = 4;
   for (int it = 0; it < max_iter; ++it) {
@@ -720,46 +972,97 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
       HZ_TRY(read_done(L.ctrl, s, &done));
       if (done) break;
     }
-    {
-      ProfScope ps(HZ_PROF_ORDER, s);
-      HZ_TRY(launch_block_order(L.pd, seed + (uint32_t)it, cats, L.pb, L.ct, L.nch_tmp,
-                                &L.ctrl->done, s));
+    // side stream: the old-row sums of this iteration's blocks (once R of the
+    // previous iteration is final), then the next iteration's block order
+    if (it >= 1) HZ_HIP(hipStreamWaitEvent(s2, ss->freed[(it - 1) & 1], 0));
+    if (use_sw) {
+      const ChunkTab& t = tabs[it & 1];
+      ia.order = t.order;
+      ia.seg_ch = t.seg_ch;
+      ia.ch_start = t.ch_start;
+      ia.ch_len = t.ch_len;
+      ia.ch_seg = t.ch_seg;
+      ProfScope ps(HZ_PROF_ORDER, s2);
+      HZ_TRY(fill<long long>(L.acc_in, nseg * L.K, 0, s2));
+      HZ_TRY(launch_insum(L, ia, s2));
     }
-    HZ_TRY(fill<long long>(L.acc_in, nseg * L.K, 0, s));
-    {
-      ProfScope ps(HZ_PROF_CENTROID, s);
-      if (use_cm)
-        HZ_TRY(launch_cent_mfma(L, cm, L.Gm, s));
-      else
-        HZ_TRY(launch_centroid<T>(L, ca, L.G, s));
-    }
-    {
-      ProfScope ps(HZ_PROF_YREDUCE, s);
-      hipLaunchKernelGGL(yreduce_kernel<T>, dim3(ygrid), dim3(kBlock), 0, s, ya);
-      HZ_LAUNCHED("yreduce_kernel");
-      if (use_rm) HZ_TRY(launch_split(L, L.ctrl, s));
-    }
-    for (int j = 0; j < L.pd.nblk; ++j) {
-      const int rows_j = std::min(L.bs, L.n - j * L.bs);
-      const int grid = (int)(ceil_div(rows_j, L.chunk) + L.B);
-      ProfScope ps(HZ_PROF_ROW, s);
-      if (use_rm) {
-        rm.seg_base = j * L.B;
-        rm.block = j;
-        rm.counter = L.counters + j;
-        HZ_TRY(launch_row_mfma(L, rm, (int)ceil_div(grid, kBlock / kWave), s));
+    HZ_HIP(hipEventRecord(ss->ready[it & 1], s2));
+    if (it + 1 < max_iter) HZ_TRY(enqueue_order(it + 1));  // table (it+1)&1 was freed above
+    bind_tab(tabs[it & 1]);
+    bool iter_done = false;
+    if (use_sw) {
+      // sweep path: centroids in natural cell order
+      {
+        ProfScope ps(HZ_PROF_CENTROID, s);
+        HZ_TRY(launch_cent_stream(L, cs, gs, s));
+      }
+      {
+        ProfScope ps(HZ_PROF_YREDUCE, s);
+        ya.G = gs;
+        ya.setup_block0 = 0;
+        hipLaunchKernelGGL(yreduce_kernel<T>, dim3(ygrid), dim3(kBlock), 0, s, ya);
+        HZ_LAUNCHED("yreduce_kernel");
+        HZ_TRY(launch_split(L, L.ctrl, s));
+      }
+      HZ_HIP(hipStreamWaitEvent(s, ss->ready[it & 1], 0));
+      int rc;
+      {
+        ProfScope ps(HZ_PROF_SWEEP, s);
+        rc = launch_sweep(L, sw, s);
+      }
+      if (rc == HZ_OK) {
+        iter_done = true;
       } else {
-        ra.seg_base = j * L.B;
-        ra.block = j;
-        ra.counter = L.counters + j;
-        HZ_TRY(launch_row<T>(L, ra, grid, s));
+        if (rc != HZ_ERR_UNSUPPORTED) return rc;
+        // not co-resident here: redo this iteration with one launch per block
+        use_sw = false;
       }
     }
+    if (!iter_done) {
+      HZ_HIP(hipStreamWaitEvent(s, ss->ready[it & 1], 0));
+      HZ_TRY(fill<long long>(L.acc_in, nseg * L.K, 0, s));
+      {
+        ProfScope ps(HZ_PROF_CENTROID, s);
+        if (use_cm)
+          HZ_TRY(launch_cent_mfma(L, cm, L.Gm, s));
+        else
+          HZ_TRY(launch_centroid<T>(L, ca, L.G, s));
+      }
+      {
+        ProfScope ps(HZ_PROF_YREDUCE, s);
+        ya.G = use_cm ? L.Gm : L.G;
+        ya.setup_block0 = 1;
+        hipLaunchKernelGGL(yreduce_kernel<T>, dim3(ygrid), dim3(kBlock), 0, s, ya);
+        HZ_LAUNCHED("yreduce_kernel");
+        if (use_rm) HZ_TRY(launch_split(L, L.ctrl, s));
+      }
+      for (int j = 0; j < L.pd.nblk; ++j) {
+        const int rows_j = std::min(L.bs, L.n - j * L.bs);
+        const int grid = (int)(ceil_div(rows_j, L.chunk) + L.B);
+        ProfScope ps(HZ_PROF_ROW, s);
+        if (use_rm) {
+          rm.seg_base = j * L.B;
+          rm.block = j;
+          rm.counter = L.counters + j;
+          HZ_TRY(launch_row_mfma(L, rm, (int)ceil_div(grid, kBlock / kWave), s));
+        } else {
+          ra.seg_base = j * L.B;
+          ra.block = j;
+          ra.counter = L.counters + j;
+          HZ_TRY(launch_row<T>(L, ra, grid, s));
+        }
+      }
+    }
+    HZ_HIP(hipEventRecord(ss->freed[it & 1], s));
   }
+  // the caller's stream waits for the side stream's last table
+  HZ_HIP(hipEventRecord(ss->done, s2));
+  HZ_HIP(hipStreamWaitEvent(s, ss->done, 0));
   Ctrl h;
   HZ_HIP(hipMemcpyAsync(&h, L.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s));
   HZ_HIP(hipStreamSynchronize(s));
   if (n_iter) *n_iter = h.n_obj;
+  HZ_RET_IF(h.pad0 != 0, HZ_ERR_LAUNCH, "block sweep: a grid barrier wait timed out");
   return HZ_OK;
 }
 

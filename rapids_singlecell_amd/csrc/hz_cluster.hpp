@@ -28,7 +28,7 @@ constexpr int kZtStride = 68;  // padded row stride of the transposed Z tile in 
 struct Ctrl {
   int done;
   int n_obj;
-  int pad0;
+  int pad0;  // set when a bounded grid-barrier wait ran out
   int pad1;
   double hist[4];
 };
@@ -387,6 +387,7 @@ struct YRedArgs {
   T* Y;
   T* YT;
   int G, K, d, Kp;
+  int setup_block0;  // remove block 0's old rows from O/E and compute its penalty
   EpiArgs<T> epi;
 };
 
@@ -448,8 +449,10 @@ __global__ __launch_bounds__(kBlock) void yreduce_kernel(YRedArgs<T> a) {
       const int c = e / (a.Kp - a.K), kk = a.K + e % (a.Kp - a.K);
       a.YT[(long long)c * a.Kp + kk] = T(0);
     }
-    __syncthreads();
-    block_update(a.epi, smem, kYRedLds, 0, 0, kUpdIn | kUpdPen, red);
+    if (a.setup_block0) {
+      __syncthreads();
+      block_update(a.epi, smem, kYRedLds, 0, 0, kUpdIn | kUpdPen, red);
+    }
   }
 }
 
@@ -660,6 +663,7 @@ __global__ void ctrl_reset_kernel(Ctrl* c) {
   if (threadIdx.x == 0) {
     c->done = 0;
     c->n_obj = 0;
+    c->pad0 = 0;
     for (int i = 0; i < 4; ++i) c->hist[i] = 0.0;
   }
 }

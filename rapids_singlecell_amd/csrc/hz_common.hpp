@@ -132,7 +132,7 @@ static inline int bit_length(unsigned long long v) {
 }
 
 static inline long long ceil_div(long long a, long long b) { return (a + b - 1) / b; }
-static inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+__host__ __device__ inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 // Bump allocator over a caller-provided workspace (256-byte aligned slices).
 struct Carve {

@@ -534,4 +534,190 @@ __global__ __launch_bounds__(kBlock) void centroid_mfma_kernel(CentMArgs a) {
   }
 }
 
+// ---------------------------------------------------------------- streaming centroid pass
+
+struct CentSArgs {
+  const float* Z;
+  const float* R;
+  float* Ypart;  // gridDim.x x K x d
+  const Ctrl* ctrl;
+  int n, d, K;
+};
+
+// Y_partial = R^T Z over 64-row tiles in natural cell order (the sum over all
+// cells does not depend on the block order).  Consecutive rows are contiguous
+// in memory, so a tile of R (64*K floats) and of Z (64*d floats) is read with
+// 16-byte loads; the next tile's loads are in flight while the current one is
+// multiplied from LDS.  Tiles are taken grid-stride; each wave owns a quarter
+// of the NT x NU output tiles.
+template <int PREC, int NT, int NU>
+__global__ __launch_bounds__(kBlock) void centroid_stream_kernel(CentSArgs a) {
+  constexpr int TR = 64;
+  constexpr int NTILE = NT * NU;
+  constexpr int MT = (NTILE + 3) / 4;
+  constexpr int MR4 = (TR * 16 * NT / 4 + kBlock - 1) / kBlock;  // float4 loads per thread (R)
+  constexpr int MZ4 = (TR * 16 * NU / 4 + kBlock - 1) / kBlock;  // float4 loads per thread (Z)
+  constexpr int SR = lds_stride(16 * NT), SZ = lds_stride(16 * NU), ST = lds_stride(TR);
+  extern __shared__ __align__(16) unsigned char smem[];
+  if (a.ctrl->done) return;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
+  const int lr = lane & 15, lg = lane >> 4;
+  const int d = a.d, K = a.K, n = a.n;
+  const int ntiles = (n + TR - 1) / TR;
+  const float invK = 1.f / (float)K, invD = 1.f / (float)d;
+
+  float* Rs = reinterpret_cast<float*>(smem);
+  float* Zs = Rs + TR * SR;
+  __bf16* RTh = reinterpret_cast<__bf16*>(smem);
+  __bf16* RTl = RTh + 16 * NT * ST;
+  __bf16* ZTh = RTl + 16 * NT * ST;
+  __bf16* ZTl = ZTh + 16 * NU * ST;
+  // padding columns (k >= K, c >= d) are never written below: clear them once
+  {
+    const size_t words = centroid_tile_lds<PREC, NT, NU>() / 4;
+    unsigned* w = reinterpret_cast<unsigned*>(smem);
+    for (size_t i = tid; i < words; i += kBlock) w[i] = 0u;
+  }
+
+  float4 rr[MR4], zz[MZ4];
+  auto load = [&](int t) {
+    const long long r0 = (long long)t * TR;
+    const int rows = (int)min((long long)TR, (long long)n - r0);
+    const float* Rb = a.R + r0 * K;
+    const float* Zb = a.Z + r0 * d;
+    const int lenR = rows * K, lenZ = rows * d;
+#pragma unroll
+    for (int m = 0; m < MR4; ++m) {
+      const int f = 4 * (tid + m * kBlock);
+      if (f + 3 < lenR) {
+        rr[m] = *reinterpret_cast<const float4*>(Rb + f);
+      } else {
+        rr[m].x = (f < lenR) ? Rb[f] : 0.f;
+        rr[m].y = (f + 1 < lenR) ? Rb[f + 1] : 0.f;
+        rr[m].z = (f + 2 < lenR) ? Rb[f + 2] : 0.f;
+        rr[m].w = 0.f;
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < MZ4; ++m) {
+      const int f = 4 * (tid + m * kBlock);
+      if (f + 3 < lenZ) {
+        zz[m] = *reinterpret_cast<const float4*>(Zb + f);
+      } else {
+        zz[m].x = (f < lenZ) ? Zb[f] : 0.f;
+        zz[m].y = (f + 1 < lenZ) ? Zb[f + 1] : 0.f;
+        zz[m].z = (f + 2 < lenZ) ? Zb[f + 2] : 0.f;
+        zz[m].w = 0.f;
+      }
+    }
+  };
+  // flat element e of a tile -> (row, column); exact for e < 2^24 / K
+  auto put = [&](int e, float v, bool isR) {
+    const int w = isR ? K : d;
+    const int row = (int)(((float)e + 0.5f) * (isR ? invK : invD));
+    const int col = e - row * w;
+    if (row >= TR) return;
+    if (PREC == kPrecF32) {
+      if (isR)
+        Rs[row * SR + col] = v;
+      else
+        Zs[row * SZ + col] = v;
+    } else {
+      const __bf16 h = (__bf16)v;
+      const __bf16 l = (__bf16)(v - (float)h);
+      if (isR) {
+        RTh[col * ST + row] = h;
+        RTl[col * ST + row] = l;
+      } else {
+        ZTh[col * ST + row] = h;
+        ZTl[col * ST + row] = l;
+      }
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int m = 0; m < MR4; ++m) {
+      const int f = 4 * (tid + m * kBlock);
+      if (f < TR * K) {
+        put(f, rr[m].x, true);
+        put(f + 1, rr[m].y, true);
+        put(f + 2, rr[m].z, true);
+        put(f + 3, rr[m].w, true);
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < MZ4; ++m) {
+      const int f = 4 * (tid + m * kBlock);
+      if (f < TR * d) {
+        put(f, zz[m].x, false);
+        put(f + 1, zz[m].y, false);
+        put(f + 2, zz[m].z, false);
+        put(f + 3, zz[m].w, false);
+      }
+    }
+  };
+
+  floatx4 acc[MT];
+#pragma unroll
+  for (int j = 0; j < MT; ++j) acc[j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  int t = blockIdx.x;
+  if (t < ntiles) load(t);
+  for (; t < ntiles; t += gridDim.x) {
+    __syncthreads();  // previous tile's LDS reads are done (and the pad clear)
+    store();
+    __syncthreads();
+    if (t + (int)gridDim.x < ntiles) load(t + gridDim.x);
+    if (PREC == kPrecF32) {
+#pragma unroll 4
+      for (int q = 0; q < TR / 4; ++q) {
+#pragma unroll
+        for (int j = 0; j < MT; ++j) {
+          const int i = wv + 4 * j;
+          if (i < NTILE) {
+            const int tt = i % NT, u = i / NT;
+            const float av = Rs[(4 * q + lg) * SR + 16 * tt + lr];
+            const float bv = Zs[(4 * q + lg) * SZ + 16 * u + lr];
+            acc[j] = mfma_f32(av, bv, acc[j]);
+          }
+        }
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < TR / 32; ++q) {
+#pragma unroll
+        for (int j = 0; j < MT; ++j) {
+          const int i = wv + 4 * j;
+          if (i < NTILE) {
+            const int tt = i % NT, u = i / NT;
+            const int ra = (16 * tt + lr) * ST + 32 * q + 8 * lg;
+            const int rb = (16 * u + lr) * ST + 32 * q + 8 * lg;
+            const bf16x8 ah = *reinterpret_cast<const bf16x8*>(RTh + ra);
+            const bf16x8 al = *reinterpret_cast<const bf16x8*>(RTl + ra);
+            const bf16x8 bh = *reinterpret_cast<const bf16x8*>(ZTh + rb);
+            const bf16x8 bl = *reinterpret_cast<const bf16x8*>(ZTl + rb);
+            acc[j] = mfma_bf16(ah, bh, acc[j]);
+            acc[j] = mfma_bf16(ah, bl, acc[j]);
+            acc[j] = mfma_bf16(al, bh, acc[j]);
+          }
+        }
+      }
+    }
+  }
+  float* out = a.Ypart + (long long)blockIdx.x * K * d;
+#pragma unroll
+  for (int j = 0; j < MT; ++j) {
+    const int i = wv + 4 * j;
+    if (i < NTILE) {
+      const int tt = i % NT, u = i / NT;
+      const int cc = 16 * u + lr;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int k = 16 * tt + 4 * lg + r;
+        if (k < K && cc < d) out[(long long)k * d + cc] = acc[j][r];
+      }
+    }
+  }
+}
+
 }  // namespace hz

@@ -71,6 +71,16 @@ static inline PermDims perm_dims(int n, int bs, int nb, int chunk) {
   return p;
 }
 
+static inline void carve_tab(Carve& c, const PermDims& p, ChunkTab& t) {
+  const long long nseg = (long long)p.nblk * p.nb;
+  t.order = c.take<int>(p.n);
+  t.seg_start = c.take<int>(nseg + 1);
+  t.seg_ch = c.take<int>(nseg + 1);
+  t.ch_start = c.take<int>(p.maxch);
+  t.ch_len = c.take<int>(p.maxch);
+  t.ch_seg = c.take<int>(p.maxch);
+}
+
 static inline void carve_perm(Carve& c, const PermDims& p, PermBufs& b, ChunkTab& t) {
   const long long nbk = 1LL << p.nbk_bits;
   const long long nseg = (long long)p.nblk * p.nb;

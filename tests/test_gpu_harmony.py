@@ -270,3 +270,57 @@ def test_kmeans_matches_oracle(device, dtype):
     Yr = ho.kmeans_centroids(Z, 8, 77)
     tol = 1e-4 if dtype == np.float32 else 1e-8
     np.testing.assert_allclose(Yg, Yr, atol=tol)
+
+
+@pytest.fixture
+def tuning():
+    """Restores the library's tuning knobs after a test changes them."""
+    before = _native.get_tuning()
+    yield _native.set_tuning
+    _native.set_tuning(**before)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("stabilized", [True, False])
+def test_clustering_sweep_vs_per_block(device, tuning, precision, stabilized):
+    """FP32 storage: the one-launch block sweep and the one-launch-per-block
+    path run the same iteration; both agree with the FP64 oracle."""
+    pb = Problem(n=12000, d=30, B=5, K=24, n_types=8, dtype=np.float32, dirichlet=True)
+    lib = _native.load()
+    dm = _native.dims(pb.n, pb.d, pb.K, pb.B, int(pb.n * 0.05), 4, precision)
+    nb = lib.hz_workspace_bytes(ctypes.byref(dm))
+    assert nb > 0, lib.hz_last_error()
+    Z64 = pb.Z_norm.astype(np.float64)
+    Rr, Er, Or, _ = ho.init_clusters(Z64, pb.Y0.astype(np.float64), sigma=0.1,
+                                     Pr_b=pb.Pr_b.astype(np.float64),
+                                     theta=pb.theta.astype(np.float64), cats=pb.codes,
+                                     n_batches=pb.B, stabilized=stabilized)
+    H = Hold(device)
+    out = {}
+    for no_sweep in (0, 1):
+        tuning(no_sweep=no_sweep)
+        ws = torch.empty(nb, dtype=torch.uint8, device=device)
+        R = _t(Rr.astype(np.float32), device)
+        E = _t(Er.astype(np.float32), device)
+        O = _t(Or.astype(np.float32), device)
+        last = torch.zeros(1, dtype=torch.float32, device=device)
+        n_iter = ctypes.c_int(0)
+        _native.check(lib.hz_clustering_loop(ctypes.byref(dm), H(pb.Z_norm), _p(R), _p(E), _p(O),
+                                             H(pb.Pr_b.ravel()), H(pb.codes), H(pb.theta), 0.1,
+                                             float("-inf"), 4, 99, int(stabilized), _p(last),
+                                             ctypes.byref(n_iter), _p(ws), nb, _s(device)),
+                      "loop")
+        assert n_iter.value == 4
+        out[no_sweep] = (R.cpu().numpy(), O.cpu().numpy(), E.cpu().numpy(), float(last.item()))
+    last_r, _, _ = ho.clustering_loop(Z64, Rr, Er, Or, pb.Pr_b.astype(np.float64), pb.codes,
+                                      pb.theta.astype(np.float64), sigma=0.1,
+                                      tol=float("-inf"), max_iter=4,
+                                      block_size=int(pb.n * 0.05), seed=99,
+                                      stabilized=stabilized)
+    for no_sweep in (0, 1):
+        R, O, E, last = out[no_sweep]
+        assert rel(R, Rr) < 1e-3, (no_sweep, rel(R, Rr))
+        assert rel(O, Or) < 1e-4
+        assert rel(E, Er) < 1e-4
+        np.testing.assert_allclose(last, last_r, rtol=1e-4)
+    assert rel(out[0][0], out[1][0]) < 1e-4
