@@ -171,6 +171,7 @@ struct Layout {
   Ctrl* ctrl;
   Ctrl* ctrl_static;  // never set done: used by the correction pass
   int* counters;      // nblk + 6
+  int* swcnt;         // nblk * B * kSweepRep: block-sweep contribution counts
   void* Ypart;
   void* Y;
   void* YT;
@@ -287,6 +288,7 @@ static int make_layout(const hz_dims* dm, void* base, Layout& L) {
   L.ctrl = c.take<Ctrl>(1);
   L.ctrl_static = c.take<Ctrl>(1);
   L.counters = c.take<int>(L.pd.nblk + 6);
+  L.swcnt = c.take<int>((long long)L.pd.nblk * L.B * kSweepRep);
   L.Ypart = c.take<char>((size_t)L.G * KD * L.esz);
   L.Y = c.take<char>((size_t)KD * L.esz);
   L.YT = c.take<char>((size_t)L.d * L.Kp * L.esz);
@@ -460,10 +462,12 @@ static bool g_sweep_broken[64];  // cooperative launch refused on this device
 
 // Returns HZ_OK after a launch, HZ_ERR_UNSUPPORTED (nothing launched) when the
 // grid cannot be made co-resident; other codes are errors.
+// a == nullptr: only choose the grid (*G, co-resident workgroups); else launch
+// with *G workgroups.
 template <int PREC, int NT, int SM>
-static int launch_sw(const Layout& L, const SweepArgs& a, hipStream_t s) {
+static int launch_sw(const Layout& L, const SweepArgs* a, int* Gp, hipStream_t s) {
   auto kern = sweep_kernel<PREC, NT, SM>;
-  const size_t lds = sweep_lds<PREC, SM>(L.Kp, L.B, L.K);
+  const size_t lds = sweep_lds<PREC, SM>(L.Kp, L.B, L.K, L.pd.nblk);
   if (lds > (size_t)device_lds_limit()) return HZ_ERR_UNSUPPORTED;
   if (lds > 65536)
     HZ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
@@ -472,11 +476,14 @@ static int launch_sw(const Layout& L, const SweepArgs& a, hipStream_t s) {
   HZ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern),
                                                       kBlock, lds));
   if (per_cu < 1) return HZ_ERR_UNSUPPORTED;
-  const long long cap = (long long)per_cu * device_cu_count();
-  long long G = g_tune[HZ_TUNE_SWEEP_WG] > 0 ? g_tune[HZ_TUNE_SWEEP_WG] : device_cu_count();
-  G = std::max<long long>(1, std::min(G, cap));
-  void* args[] = {const_cast<SweepArgs*>(&a)};
-  hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kern), dim3((unsigned)G),
+  if (!a) {
+    const long long cap = (long long)per_cu * device_cu_count();
+    long long G = g_tune[HZ_TUNE_SWEEP_WG] > 0 ? g_tune[HZ_TUNE_SWEEP_WG] : device_cu_count();
+    *Gp = (int)std::max<long long>(1, std::min(G, cap));
+    return HZ_OK;
+  }
+  void* args[] = {const_cast<SweepArgs*>(a)};
+  hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kern), dim3((unsigned)*Gp),
                                             dim3(kBlock), args, (unsigned)lds, s);
   if (e != hipSuccess) {
     (void)hipGetLastError();
@@ -487,12 +494,12 @@ static int launch_sw(const Layout& L, const SweepArgs& a, hipStream_t s) {
 }
 
 template <int PREC, int SM>
-static int launch_sw_sm(const Layout& L, const SweepArgs& a, hipStream_t s) {
+static int launch_sw_sm(const Layout& L, const SweepArgs* a, int* G, hipStream_t s) {
   switch (L.NT) {
-    case 4: return launch_sw<PREC, 4, SM>(L, a, s);
-    case 7: return launch_sw<PREC, 7, SM>(L, a, s);
-    case 8: return launch_sw<PREC, 8, SM>(L, a, s);
-    default: return launch_sw<PREC, 16, SM>(L, a, s);
+    case 4: return launch_sw<PREC, 4, SM>(L, a, G, s);
+    case 7: return launch_sw<PREC, 7, SM>(L, a, G, s);
+    case 8: return launch_sw<PREC, 8, SM>(L, a, G, s);
+    default: return launch_sw<PREC, 16, SM>(L, a, G, s);
   }
 }
 
@@ -509,22 +516,22 @@ static int launch_insum(const Layout& L, const InSumArgs& a, hipStream_t s) {
   return HZ_OK;
 }
 
-static int launch_sweep(const Layout& L, const SweepArgs& a, hipStream_t s) {
+static int launch_sweep(const Layout& L, const SweepArgs* a, int* G, hipStream_t s) {
   if (L.prec == HZ_PREC_SPLIT_BF16) {
     switch (row_sm(L)) {
-      case 1: return launch_sw_sm<kPrecBF16, 1>(L, a, s);
-      case 2: return launch_sw_sm<kPrecBF16, 2>(L, a, s);
-      case 3: return launch_sw_sm<kPrecBF16, 3>(L, a, s);
-      default: return launch_sw_sm<kPrecBF16, 4>(L, a, s);
+      case 1: return launch_sw_sm<kPrecBF16, 1>(L, a, G, s);
+      case 2: return launch_sw_sm<kPrecBF16, 2>(L, a, G, s);
+      case 3: return launch_sw_sm<kPrecBF16, 3>(L, a, G, s);
+      default: return launch_sw_sm<kPrecBF16, 4>(L, a, G, s);
     }
   }
   switch (row_sm(L)) {
-    case 4: return launch_sw_sm<kPrecF32, 4>(L, a, s);
-    case 8: return launch_sw_sm<kPrecF32, 8>(L, a, s);
-    case 13: return launch_sw_sm<kPrecF32, 13>(L, a, s);
-    case 16: return launch_sw_sm<kPrecF32, 16>(L, a, s);
-    case 25: return launch_sw_sm<kPrecF32, 25>(L, a, s);
-    default: return launch_sw_sm<kPrecF32, 32>(L, a, s);
+    case 4: return launch_sw_sm<kPrecF32, 4>(L, a, G, s);
+    case 8: return launch_sw_sm<kPrecF32, 8>(L, a, G, s);
+    case 13: return launch_sw_sm<kPrecF32, 13>(L, a, G, s);
+    case 16: return launch_sw_sm<kPrecF32, 16>(L, a, G, s);
+    case 25: return launch_sw_sm<kPrecF32, 25>(L, a, G, s);
+    default: return launch_sw_sm<kPrecF32, 32>(L, a, G, s);
   }
 }
 
@@ -532,7 +539,7 @@ template <int PREC, int NT, int NU>
 static int launch_cs(const CentSArgs& a, int grid, hipStream_t s) {
   if constexpr (NT * NU <= 32) {
     auto kern = centroid_stream_kernel<PREC, NT, NU>;
-    const size_t lds = centroid_tile_lds<PREC, NT, NU>();
+    const size_t lds = stream_tile_lds<NT, NU>();
     HZ_RET_IF(lds > (size_t)device_lds_limit(), HZ_ERR_UNSUPPORTED,
               "centroid pass needs %zu bytes of LDS", lds);
     if (lds > 65536)
@@ -857,6 +864,7 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
     cm.scale_o = L.scale_o;
   }
   bool use_sw = false;
+  int sweep_grid = 1;  // workgroups of the block sweep (all co-resident)
   SweepArgs sw{};
   CentSArgs cs{};
   int gs = 1;
@@ -864,7 +872,8 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
     // 16-byte loads of R and Z rows in the streaming centroid pass
     const bool aligned = (reinterpret_cast<uintptr_t>(Z) % 16 == 0) &&
                          (reinterpret_cast<uintptr_t>(R) % 16 == 0);
-    use_sw = use_rm && use_cm && aligned && L.K <= 256 && g_tune[HZ_TUNE_SWEEP] == 0 &&
+    use_sw = use_rm && use_cm && aligned && L.K <= 256 && L.pd.nblk <= kSweepMaxBlocks &&
+             g_tune[HZ_TUNE_SWEEP] == 0 &&
              !g_sweep_broken[current_device()];
     sw.Z = Z;
     sw.YT = reinterpret_cast<const float*>(L.YT);
@@ -886,9 +895,7 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
     sw.theta = theta;
     sw.ctrl = L.ctrl;
     sw.last_obj = last_obj;
-    sw.bar = L.counters + L.pd.nblk + 2;
     sw.fin = L.counters + L.pd.nblk + 3;
-    sw.flag = L.counters + L.pd.nblk + 4;
     sw.n = L.n;
     sw.d = L.d;
     sw.K = L.K;
@@ -904,7 +911,11 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
     sw.inv_scale_o = 1.0 / L.scale_o;
     sw.scale_obj = L.scale_obj;
     sw.inv_scale_obj = 1.0 / L.scale_obj;
-    sw.spin_limit = 1u << 20;
+    sw.cnt = L.swcnt;
+    // counted block sums: value field below bit 53 holds at most N * 2^s
+    sw.scale_pk = std::ldexp(1.0, std::min(40, 52 - bit_length((unsigned long long)L.n + 1)));
+    sw.inv_scale_pk = 1.0 / sw.scale_pk;
+    sw.spin_limit = 1u << 16;  // polls of about a microsecond: a wait that long is a fault
     cs.Z = Z;
     cs.R = R;
     cs.Ypart = reinterpret_cast<float*>(L.Ypart);
@@ -913,6 +924,7 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
     cs.d = L.d;
     cs.K = L.K;
     gs = (int)std::max<long long>(1, std::min<long long>(L.Gm, ceil_div(L.n, 64)));
+    if (use_sw && launch_sweep(L, nullptr, &sweep_grid, s) != HZ_OK) use_sw = false;
   }
 
   // block orders are built one iteration ahead on a side stream, into two
@@ -985,6 +997,10 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
       ProfScope ps(HZ_PROF_ORDER, s2);
       HZ_TRY(fill<long long>(L.acc_in, nseg * L.K, 0, s2));
       HZ_TRY(launch_insum(L, ia, s2));
+      hipLaunchKernelGGL(sweep_count_kernel, dim3(L.pd.nblk), dim3(kBlock),
+                         (size_t)L.B * kSweepRep * sizeof(int), s2, t.seg_ch, t.ch_seg, L.swcnt,
+                         L.B, sweep_grid);
+      HZ_LAUNCHED("sweep_count_kernel");
     }
     HZ_HIP(hipEventRecord(ss->ready[it & 1], s2));
     if (it + 1 < max_iter) HZ_TRY(enqueue_order(it + 1));  // table (it+1)&1 was freed above
@@ -1008,7 +1024,7 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
       int rc;
       {
         ProfScope ps(HZ_PROF_SWEEP, s);
-        rc = launch_sweep(L, sw, s);
+        rc = launch_sweep(L, &sw, &sweep_grid, s);
       }
       if (rc == HZ_OK) {
         iter_done = true;

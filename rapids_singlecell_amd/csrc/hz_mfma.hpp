@@ -548,16 +548,27 @@ struct CentSArgs {
 // cells does not depend on the block order).  Consecutive rows are contiguous
 // in memory, so a tile of R (64*K floats) and of Z (64*d floats) is read with
 // 16-byte loads; the next tile's loads are in flight while the current one is
-// multiplied from LDS.  Tiles are taken grid-stride; each wave owns a quarter
-// of the NT x NU output tiles.
+// multiplied from LDS.  Tiles are taken grid-stride.
+//
+// Both precisions stage the tiles as FP32, row-major.  Wave w owns the output
+// column tiles u = u0, u0 + NG, ... and row tiles t = t0, t0 + TS, ..., so one
+// k-step needs MTT A fragments and MU B fragments, read in one batch before
+// its MFMAs.  Split-BF16 splits the fragments after the read.
+template <int NT, int NU>
+__host__ __device__ constexpr size_t stream_tile_lds() {
+  return (size_t)64 * (lds_stride(16 * NT) + lds_stride(16 * NU)) * 4;
+}
+
 template <int PREC, int NT, int NU>
 __global__ __launch_bounds__(kBlock) void centroid_stream_kernel(CentSArgs a) {
   constexpr int TR = 64;
-  constexpr int NTILE = NT * NU;
-  constexpr int MT = (NTILE + 3) / 4;
+  constexpr int NG = NU < 4 ? NU : 4;        // wave groups along u
+  constexpr int TS = 4 / NG;                 // waves per group (split along t)
+  constexpr int MU = (NU + NG - 1) / NG;     // u tiles per wave
+  constexpr int MTT = (NT + TS - 1) / TS;    // t tiles per wave
   constexpr int MR4 = (TR * 16 * NT / 4 + kBlock - 1) / kBlock;  // float4 loads per thread (R)
   constexpr int MZ4 = (TR * 16 * NU / 4 + kBlock - 1) / kBlock;  // float4 loads per thread (Z)
-  constexpr int SR = lds_stride(16 * NT), SZ = lds_stride(16 * NU), ST = lds_stride(TR);
+  constexpr int SR = lds_stride(16 * NT), SZ = lds_stride(16 * NU);
   extern __shared__ __align__(16) unsigned char smem[];
   if (a.ctrl->done) return;
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
@@ -565,22 +576,16 @@ __global__ __launch_bounds__(kBlock) void centroid_stream_kernel(CentSArgs a) {
   const int d = a.d, K = a.K, n = a.n;
   const int ntiles = (n + TR - 1) / TR;
   const float invK = 1.f / (float)K, invD = 1.f / (float)d;
+  const bool active = wv < NG * TS;
+  const int u0 = wv % NG, t0 = wv / NG;
 
   float* Rs = reinterpret_cast<float*>(smem);
   float* Zs = Rs + TR * SR;
-  __bf16* RTh = reinterpret_cast<__bf16*>(smem);
-  __bf16* RTl = RTh + 16 * NT * ST;
-  __bf16* ZTh = RTl + 16 * NT * ST;
-  __bf16* ZTl = ZTh + 16 * NU * ST;
   // padding columns (k >= K, c >= d) are never written below: clear them once
-  {
-    const size_t words = centroid_tile_lds<PREC, NT, NU>() / 4;
-    unsigned* w = reinterpret_cast<unsigned*>(smem);
-    for (size_t i = tid; i < words; i += kBlock) w[i] = 0u;
-  }
+  for (int i = tid; i < TR * (SR + SZ); i += kBlock) Rs[i] = 0.f;
 
   float4 rr[MR4], zz[MZ4];
-  auto load = [&](int t) {
+  auto load = [&](int t) __attribute__((always_inline)) {
     const long long r0 = (long long)t * TR;
     const int rows = (int)min((long long)TR, (long long)n - r0);
     const float* Rb = a.R + r0 * K;
@@ -611,55 +616,39 @@ __global__ __launch_bounds__(kBlock) void centroid_stream_kernel(CentSArgs a) {
       }
     }
   };
-  // flat element e of a tile -> (row, column); exact for e < 2^24 / K
-  auto put = [&](int e, float v, bool isR) {
-    const int w = isR ? K : d;
-    const int row = (int)(((float)e + 0.5f) * (isR ? invK : invD));
-    const int col = e - row * w;
-    if (row >= TR) return;
-    if (PREC == kPrecF32) {
-      if (isR)
-        Rs[row * SR + col] = v;
-      else
-        Zs[row * SZ + col] = v;
-    } else {
-      const __bf16 h = (__bf16)v;
-      const __bf16 l = (__bf16)(v - (float)h);
-      if (isR) {
-        RTh[col * ST + row] = h;
-        RTl[col * ST + row] = l;
-      } else {
-        ZTh[col * ST + row] = h;
-        ZTl[col * ST + row] = l;
-      }
+  // flat element e of a tile -> (row, column); exact for e < 2^24 / width.
+  // A float4 whose four elements share a row (width % 4 == 0) is one store.
+  auto put4 = [&](int f, const float4& v, float* S, int stride, int w, float invw, bool vec)
+      __attribute__((always_inline)) {
+    if (f >= TR * w) return;
+    const int row = (int)(((float)f + 0.5f) * invw);
+    const int col = f - row * w;
+    if (vec) {
+      *reinterpret_cast<float4*>(S + row * stride + col) = v;
+      return;
     }
+    const float x[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int e = f + i;
+      const int r2 = (int)(((float)e + 0.5f) * invw);
+      if (r2 < TR) S[r2 * stride + (e - r2 * w)] = x[i];
+    }
+    (void)col;
   };
-  auto store = [&]() {
+  const bool vecR = (K % 4) == 0, vecZ = (d % 4) == 0;
+  auto store = [&]() __attribute__((always_inline)) {
 #pragma unroll
-    for (int m = 0; m < MR4; ++m) {
-      const int f = 4 * (tid + m * kBlock);
-      if (f < TR * K) {
-        put(f, rr[m].x, true);
-        put(f + 1, rr[m].y, true);
-        put(f + 2, rr[m].z, true);
-        put(f + 3, rr[m].w, true);
-      }
-    }
+    for (int m = 0; m < MR4; ++m) put4(4 * (tid + m * kBlock), rr[m], Rs, SR, K, invK, vecR);
 #pragma unroll
-    for (int m = 0; m < MZ4; ++m) {
-      const int f = 4 * (tid + m * kBlock);
-      if (f < TR * d) {
-        put(f, zz[m].x, false);
-        put(f + 1, zz[m].y, false);
-        put(f + 2, zz[m].z, false);
-        put(f + 3, zz[m].w, false);
-      }
-    }
+    for (int m = 0; m < MZ4; ++m) put4(4 * (tid + m * kBlock), zz[m], Zs, SZ, d, invD, vecZ);
   };
 
-  floatx4 acc[MT];
+  floatx4 acc[MU][MTT];
 #pragma unroll
-  for (int j = 0; j < MT; ++j) acc[j] = floatx4{0.f, 0.f, 0.f, 0.f};
+  for (int i = 0; i < MU; ++i)
+#pragma unroll
+    for (int j = 0; j < MTT; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
   int t = blockIdx.x;
   if (t < ntiles) load(t);
@@ -668,53 +657,83 @@ __global__ __launch_bounds__(kBlock) void centroid_stream_kernel(CentSArgs a) {
     store();
     __syncthreads();
     if (t + (int)gridDim.x < ntiles) load(t + gridDim.x);
+    if (!active) continue;
     if (PREC == kPrecF32) {
-#pragma unroll 4
-      for (int q = 0; q < TR / 4; ++q) {
+      // operands of k-step q+1 are read before the MFMAs of step q
+      float av[2][MTT], bv[2][MU];
+      auto rd = [&](int q, int bsel) __attribute__((always_inline)) {
+        const int row = 4 * q + lg;
 #pragma unroll
-        for (int j = 0; j < MT; ++j) {
-          const int i = wv + 4 * j;
-          if (i < NTILE) {
-            const int tt = i % NT, u = i / NT;
-            const float av = Rs[(4 * q + lg) * SR + 16 * tt + lr];
-            const float bv = Zs[(4 * q + lg) * SZ + 16 * u + lr];
-            acc[j] = mfma_f32(av, bv, acc[j]);
-          }
-        }
+        for (int j = 0; j < MTT; ++j)
+          av[bsel][j] = Rs[row * SR + 16 * min(t0 + j * TS, NT - 1) + lr];
+#pragma unroll
+        for (int i = 0; i < MU; ++i) bv[bsel][i] = Zs[row * SZ + 16 * min(u0 + i * NG, NU - 1) + lr];
+      };
+      rd(0, 0);
+#pragma unroll
+      for (int q = 0; q < TR / 4; ++q) {
+        if (q + 1 < TR / 4) rd(q + 1, (q + 1) & 1);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < MU; ++i)
+#pragma unroll
+          for (int j = 0; j < MTT; ++j) acc[i][j] = mfma_f32(av[q & 1][j], bv[q & 1][i], acc[i][j]);
+        __builtin_amdgcn_sched_barrier(0);
       }
     } else {
-#pragma unroll
+      // per k-step of 32 rows: B fragments once, then A fragments one t tile
+      // ahead of its three MFMAs
+#pragma unroll 1
       for (int q = 0; q < TR / 32; ++q) {
+        const int row0 = 32 * q + 8 * lg;
+        bf16x8 bh[MU], bl[MU];
 #pragma unroll
-        for (int j = 0; j < MT; ++j) {
-          const int i = wv + 4 * j;
-          if (i < NTILE) {
-            const int tt = i % NT, u = i / NT;
-            const int ra = (16 * tt + lr) * ST + 32 * q + 8 * lg;
-            const int rb = (16 * u + lr) * ST + 32 * q + 8 * lg;
-            const bf16x8 ah = *reinterpret_cast<const bf16x8*>(RTh + ra);
-            const bf16x8 al = *reinterpret_cast<const bf16x8*>(RTl + ra);
-            const bf16x8 bh = *reinterpret_cast<const bf16x8*>(ZTh + rb);
-            const bf16x8 bl = *reinterpret_cast<const bf16x8*>(ZTl + rb);
-            acc[j] = mfma_bf16(ah, bh, acc[j]);
-            acc[j] = mfma_bf16(ah, bl, acc[j]);
-            acc[j] = mfma_bf16(al, bh, acc[j]);
+        for (int i = 0; i < MU; ++i) {
+          const int col = 16 * min(u0 + i * NG, NU - 1) + lr;
+          float x[8];
+#pragma unroll
+          for (int v = 0; v < 8; ++v) x[v] = Zs[(row0 + v) * SZ + col];
+          split_bf16(x, bh[i], bl[i]);
+        }
+        float xa[2][8];
+        auto rda = [&](int j, int bsel) __attribute__((always_inline)) {
+          const int col = 16 * min(t0 + j * TS, NT - 1) + lr;
+#pragma unroll
+          for (int v = 0; v < 8; ++v) xa[bsel][v] = Rs[(row0 + v) * SR + col];
+        };
+        rda(0, 0);
+#pragma unroll
+        for (int j = 0; j < MTT; ++j) {
+          if (j + 1 < MTT) rda(j + 1, (j + 1) & 1);
+          __builtin_amdgcn_sched_barrier(0);
+          bf16x8 ah, al;
+          split_bf16(xa[j & 1], ah, al);
+#pragma unroll
+          for (int i = 0; i < MU; ++i) {
+            acc[i][j] = mfma_bf16(ah, bh[i], acc[i][j]);
+            acc[i][j] = mfma_bf16(ah, bl[i], acc[i][j]);
+            acc[i][j] = mfma_bf16(al, bh[i], acc[i][j]);
           }
+          __builtin_amdgcn_sched_barrier(0);
         }
       }
     }
   }
+  if (!active) return;
   float* out = a.Ypart + (long long)blockIdx.x * K * d;
 #pragma unroll
-  for (int j = 0; j < MT; ++j) {
-    const int i = wv + 4 * j;
-    if (i < NTILE) {
-      const int tt = i % NT, u = i / NT;
-      const int cc = 16 * u + lr;
+  for (int i = 0; i < MU; ++i) {
+    const int u = u0 + i * NG;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int k = 16 * tt + 4 * lg + r;
-        if (k < K && cc < d) out[(long long)k * d + cc] = acc[j][r];
+    for (int j = 0; j < MTT; ++j) {
+      const int tt = t0 + j * TS;
+      if (u < NU && tt < NT) {
+        const int cc = 16 * u + lr;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int k = 16 * tt + 4 * lg + r;
+          if (k < K && cc < d) out[(long long)k * d + cc] = acc[i][j][r];
+        }
       }
     }
   }

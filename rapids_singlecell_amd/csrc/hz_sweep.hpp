@@ -12,11 +12,13 @@
 //   for each block j:
 //     rows of block j: exp(term (1 - sim)) * pen[batch] -> row normalise ->
 //       store R; per-workgroup batch x cluster sums of the new rows
-//     fixed-point adds of those sums into acc_out (device-scope atomics whose
-//       results are consumed, so they are performed before the arrival)
-//     arrive at the grid barrier; while it completes, load block j+1's
-//       old-row sums and compute the similarity tile of each wave's first
-//       chunk of block j+1 (both independent of the penalty); wait
+//     adds of those sums into acc_out, each carrying a count of 1 beside the
+//       fixed-point value (non-returning device-scope atomics)
+//     while the other workgroups' sums arrive, take block j+1's old-row sums
+//       and compute the similarity tile of each wave's first chunk of block
+//       j+1 (both independent of the penalty)
+//     read block j's sums once every entry's count shows all of its
+//       contributors (no separate grid barrier: the data carries the arrival)
 //     every workgroup applies the same O/E update to its own LDS copy,
 //       O += out_j - in_{j+1};  E += Pr (x) (colsum out_j - colsum in_{j+1})
 //     and recomputes the penalty of block j+1 (identical arithmetic in every
@@ -26,8 +28,8 @@
 //
 // All workgroups must be resident at once: the host launches it with
 // hipLaunchCooperativeKernel, which refuses a grid that does not fit.  Every
-// barrier wait is bounded (`spin_limit` polls); a wait that runs out sets
-// Ctrl::pad0, which the host reports as an error, so every wave always exits.
+// wait is bounded (`spin_limit` polls); a wait that runs out sets Ctrl::pad0,
+// which the host reports as an error, so every wave always exits.
 #pragma once
 
 #include "hz_mfma.hpp"
@@ -114,6 +116,32 @@ __global__ __launch_bounds__(kBlock) void insum_kernel(InSumArgs a) {
 
 // ---------------------------------------------------------------- sweep
 
+// Number of workgroups of a G-workgroup sweep that add to each acc_out entry
+// of block j, batch b, replica r: workgroup w owns the contiguous chunk range
+// [nlo + w*per, min(nhi, nlo + (w+1)*per)) of block j and adds to every batch
+// from its first chunk's to its last chunk's, in replica w % kSweepRep (the
+// same rule as the sweep's flush).  One workgroup per block.
+__global__ __launch_bounds__(kBlock) void sweep_count_kernel(const int* __restrict__ seg_ch,
+                                                             const int* __restrict__ ch_seg,
+                                                             int* __restrict__ cnt, int B, int G) {
+  extern __shared__ int s_c[];  // B x kSweepRep
+  const int j = blockIdx.x;
+  for (int i = threadIdx.x; i < B * kSweepRep; i += blockDim.x) s_c[i] = 0;
+  __syncthreads();
+  const int nlo = seg_ch[j * B], nhi = seg_ch[j * B + B];
+  const int per = (nhi - nlo + G - 1) / G;
+  for (int w = threadIdx.x; w < G; w += blockDim.x) {
+    const int lo = nlo + w * per;
+    const int hi = min(nhi, lo + per);
+    if (lo < hi) {
+      const int bl = ch_seg[lo] - j * B, bh = ch_seg[hi - 1] - j * B;
+      for (int b = bl; b <= bh; ++b) atomicAdd(&s_c[b * kSweepRep + w % kSweepRep], 1);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < B * kSweepRep; i += blockDim.x) cnt[j * B * kSweepRep + i] = s_c[i];
+}
+
 struct SweepArgs {
   const float* Z;
   const float* YT;   // d x Kp (f32 mode)
@@ -128,6 +156,7 @@ struct SweepArgs {
   const long long* acc_in;  // nseg x K, old-row sums (complete at launch)
   long long* acc_out;       // kSweepRep x nseg x K, zero at launch
   long long* obj_acc;       // kSweepRep x 2, zero at launch
+  const int* cnt;           // nblk x B x kSweepRep contributions per acc_out entry
   long long rep_stride;
   float* O;
   float* E;
@@ -135,49 +164,36 @@ struct SweepArgs {
   const float* theta;
   Ctrl* ctrl;
   float* last_obj;
-  int* bar;   // barrier arrivals, 0 at launch, 0 again at exit
-  int* flag;  // barrier generation, 0 at launch, 0 again at exit
   int* fin;   // final arrivals, 0 at launch, 0 again at exit
   int n, d, K, Kp, dP, B, nblk;
   float term, sigma, tol;
   int stabilized;
   double scale_o, inv_scale_o, scale_obj, inv_scale_obj;
+  double scale_pk, inv_scale_pk;  // value scale of the counted block sums (acc_out)
   unsigned spin_limit;
 };
+
+// acc_out entries carry their number of contributions: each workgroup adds
+// (1 << kCountShift) + fixed-point value, so a reader knows the sum is complete
+// when the count reaches the number of workgroups that contribute to it.
+constexpr int kCountShift = 53;
+constexpr long long kValueMask = (1LL << kCountShift) - 1;
 
 // LDS of the sweep: staged centroids + O, E, pen (float B x K each) + the
 // new-row deltas (double B x K; the float per-block sums alias them), the
 // old-row deltas (double B x K) and their column sums.
+// Per-workgroup table of each wave's first chunk of every block: segment,
+// length, start and the 64 cell ids (built once at launch, so that the rows of
+// the next block can be fetched without a chain of dependent loads).
+constexpr int kTabStride = 68;  // ints per (block, wave) entry: seg, len, p0, pad, 64 cells
+constexpr int kSweepMaxBlocks = 64;
+
 template <int PREC, int SM>
-__host__ __device__ inline size_t sweep_lds(int Kp, int B, int K) {
+__host__ __device__ inline size_t sweep_lds(int Kp, int B, int K, int nblk) {
   const size_t y = align_up(row_mfma_lds<PREC, SM>(Kp), 16);
-  return y + (size_t)3 * B * K * 4 + (size_t)2 * B * K * 8 + (size_t)2 * K * 8 + 64;
-}
-
-// Split grid barrier on a generation flag: `grid_arrive` counts this
-// workgroup in (the last of the G arrivals of generation `gen` publishes it),
-// `grid_wait` polls the flag (not the contended counter).  Work placed between
-// the two overlaps the wait.  Waits are bounded.
-__device__ __forceinline__ void grid_arrive(int* bar, int* flag, int gen, int G) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const int old = __hip_atomic_fetch_add(bar, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (old == G * gen - 1) __hip_atomic_store(flag, gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-}
-
-__device__ __forceinline__ void grid_wait(const int* flag, int gen, unsigned spin_limit, int* err) {
-  if (threadIdx.x == 0) {
-    unsigned it = 0;
-    while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < gen) {
-      __builtin_amdgcn_s_sleep(1);
-      if (++it >= spin_limit) {
-        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        break;
-      }
-    }
-  }
-  __syncthreads();
+  return y + (size_t)3 * B * K * 4 + (size_t)2 * B * K * 8 + (size_t)2 * K * 8 +
+         (size_t)nblk * (kBlock / kWave) * kTabStride * 4 + (size_t)nblk * 4 * 4 +
+         (size_t)nblk * B * kSweepRep * 4 + 64;
 }
 
 template <int PREC, int NT, int SM>
@@ -191,7 +207,9 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
   __shared__ int s_seg[NW];
   __shared__ double s_obj2[NW][2];
   Ctrl* ctrl = a.ctrl;
-  if (ctrl->done) return;  // written only between launches: the same in every workgroup
+  // written only between launches, so the same in every workgroup; an earlier
+  // wait that ran out (pad0) ends the remaining launches at once
+  if (ctrl->done || ctrl->pad0) return;
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
   const int lr = lane & 15, lg = lane >> 4;
   const int d = a.d, K = a.K, Kp = a.Kp, B = a.B;
@@ -208,6 +226,9 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
   double* s_di = s_do + BK;                              // old-row deltas of the next block
   double* s_co = s_di + BK;
   double* s_ci = s_co + K;
+  int* s_tab = reinterpret_cast<int*>(s_ci + K);  // nblk x NW x kTabStride
+  int* s_lohi = s_tab + a.nblk * NW * kTabStride; // nblk x 4: chunk range, batch range
+  int* s_cnt = s_lohi + 4 * a.nblk;  // nblk x B x kSweepRep: contributions per acc_out entry
 
   // stage the normalised centroids (same padded layouts as the row kernel)
   const int KpS = Kp + 16;
@@ -247,15 +268,58 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
 
   // this workgroup's contiguous chunk range of block j (may be empty)
   auto wg_range = [&](int j, int& lo, int& hi) __attribute__((always_inline)) {
-    const int nlo = a.seg_ch[j * B], nhi = a.seg_ch[j * B + B];
-    const int per = (nhi - nlo + G - 1) / G;
-    lo = nlo + blockIdx.x * per;
-    hi = min(nhi, lo + per);
+    lo = s_lohi[4 * j];
+    hi = s_lohi[4 * j + 1];
   };
 
+  // chunk ranges and first-chunk table of every block (three rounds of
+  // independent loads, once per launch)
+  for (int j = tid; j < a.nblk; j += kBlock) {
+    const int nlo = a.seg_ch[j * B], nhi = a.seg_ch[j * B + B];
+    const int per = (nhi - nlo + G - 1) / G;
+    const int lo = nlo + blockIdx.x * per;
+    const int hi = min(nhi, lo + per);
+    s_lohi[4 * j] = lo;
+    s_lohi[4 * j + 1] = hi;
+    // batches touched (chunks are in segment order): [bl, bh], empty if bh < bl
+    s_lohi[4 * j + 2] = (lo < hi) ? a.ch_seg[lo] - j * B : 0;
+    s_lohi[4 * j + 3] = (lo < hi) ? a.ch_seg[hi - 1] - j * B : -1;
+  }
+  __syncthreads();
+  for (int j = lane; j < a.nblk; j += kWave) {
+    const int c = s_lohi[4 * j] + wv;
+    int* e = s_tab + (j * NW + wv) * kTabStride;
+    if (c < s_lohi[4 * j + 1]) {
+      e[0] = a.ch_seg[c];
+      e[1] = a.ch_len[c];
+      e[2] = a.ch_start[c];
+    } else {
+      e[0] = -1;
+      e[1] = 0;
+      e[2] = 0;
+    }
+  }
+  // expected contributions per (block, batch, replica), from sweep_count_kernel
+  for (int i = tid; i < a.nblk * B * kSweepRep; i += kBlock) s_cnt[i] = a.cnt[i];
+  for (int j0 = 0; j0 < a.nblk; j0 += 8) {
+    int cl[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int j = min(j0 + u, a.nblk - 1);
+      const int* e = s_tab + (j * NW + wv) * kTabStride;
+      const int len = e[1];
+      const int x = a.order[(lane < len) ? e[2] + lane : 0];  // order[0] is always valid
+      cl[u] = (lane < len) ? x : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (j0 + u < a.nblk) s_tab[((j0 + u) * NW + wv) * kTabStride + 4 + lane] = cl[u];
+  }
+  __syncthreads();
+
   // similarities of 16 rows (r0 .. r0+15 of a chunk) on matrix cores
-  auto sim_tile = [&](int my_cell, int len, int r0, floatx4 (&acc)[NT]) __attribute__((always_inline)) {
-    float za[NA];
+  // A operand of one 16-row tile: row r0 + lr of the chunk (zero beyond len)
+  auto load_z = [&](int my_cell, int len, int r0, float (&za)[NA]) __attribute__((always_inline)) {
     const int zc = __shfl(my_cell, (r0 + lr) & 63, kWave);
     const bool zvalid = r0 + lr < len;
     const unsigned zbase = (unsigned)(zvalid ? zc : 0) * (unsigned)d;
@@ -263,16 +327,29 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
     for (int i = 0; i < NA; ++i) {
       const int cc = (PREC == kPrecF32) ? 4 * i + lg : 32 * (i / 8) + 8 * lg + (i % 8);
       const float zv = a.Z[zbase + (unsigned)min(cc, d - 1)];
-      za[i] = (zvalid && cc < d) ? zv : 0.f;
+      za[i] = zv * ((zvalid && cc < d) ? 1.f : 0.f);
     }
+  };
+  // MFMA chain of one tile; the B fragments (centroids in LDS) of the next
+  // k-step are read before the MFMAs of the current one
+  auto sim_mfma = [&](const float (&za)[NA], floatx4 (&acc)[NT]) __attribute__((always_inline)) {
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
     if (PREC == kPrecF32) {
-#pragma unroll
-      for (int s = 0; s < SM; ++s) {
+      float bv[2][NT];
+      auto rd = [&](int s, int bsel) __attribute__((always_inline)) {
         const int cc = 4 * s + lg;
 #pragma unroll
-        for (int t = 0; t < NT; ++t) acc[t] = mfma_f32(za[s], YTs[cc * KpS + 16 * t + lr], acc[t]);
+        for (int t = 0; t < NT; ++t) bv[bsel][t] = YTs[cc * KpS + 16 * t + lr];
+      };
+      rd(0, 0);
+#pragma unroll
+      for (int s = 0; s < SM; ++s) {
+        if (s + 1 < SM) rd(s + 1, (s + 1) & 1);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[t] = mfma_f32(za[s], bv[s & 1][t], acc[t]);
+        __builtin_amdgcn_sched_barrier(0);
       }
     } else {
 #pragma unroll
@@ -282,17 +359,29 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
         for (int v = 0; v < 8; ++v) xs[v] = za[8 * s + v];
         bf16x8 ah, al;
         split_bf16(xs, ah, al);
+        bf16x8 bh[2], bl[2];
+        auto rd = [&](int t, int bsel) __attribute__((always_inline)) {
+          const int off = (16 * t + lr) * dPS + 32 * s + 8 * lg;
+          bh[bsel] = *reinterpret_cast<const bf16x8*>(YHs + off);
+          bl[bsel] = *reinterpret_cast<const bf16x8*>(YLs + off);
+        };
+        rd(0, 0);
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
-          const int off = (16 * t + lr) * dPS + 32 * s + 8 * lg;
-          const bf16x8 bh = *reinterpret_cast<const bf16x8*>(YHs + off);
-          const bf16x8 bl = *reinterpret_cast<const bf16x8*>(YLs + off);
-          acc[t] = mfma_bf16(ah, bh, acc[t]);
-          acc[t] = mfma_bf16(ah, bl, acc[t]);
-          acc[t] = mfma_bf16(al, bh, acc[t]);
+          if (t + 1 < NT) rd(t + 1, (t + 1) & 1);
+          __builtin_amdgcn_sched_barrier(0);
+          acc[t] = mfma_bf16(ah, bh[t & 1], acc[t]);
+          acc[t] = mfma_bf16(ah, bl[t & 1], acc[t]);
+          acc[t] = mfma_bf16(al, bh[t & 1], acc[t]);
+          __builtin_amdgcn_sched_barrier(0);
         }
       }
     }
+  };
+  auto sim_tile = [&](int my_cell, int len, int r0, floatx4 (&acc)[NT]) __attribute__((always_inline)) {
+    float za[NA];
+    load_z(my_cell, len, r0, za);
+    sim_mfma(za, acc);
   };
 
   // row epilogue of one 16-row tile (kernels_pen.cuh:27-82): lane holds rows 4*lg + r
@@ -315,7 +404,7 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
         v[t] = (16 * t + lr < K) ? dexp(a.term * (1.f - acc[t][r])) * pv[t] : 0.f;
         s += v[t];
       }
-      s = group_sum(s, 16);
+      s = sum16(s);
       const float inv = 1.f / s;
       float s2 = 0.f;
 #pragma unroll
@@ -323,7 +412,7 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
         v[t] *= inv;
         s2 += v[t];
       }
-      s2 = group_sum(s2, 16);
+      s2 = sum16(s2);
       const float inv2 = 1.f / s2;
       if (rr < len) {
         float* dst = a.R + (unsigned)rcell * (unsigned)K;
@@ -342,20 +431,31 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
     }
   };
 
-  // wave-carried prefetch: the first chunk of the next block
+  // wave-carried prefetch of the first chunk of the next block: its table
+  // entry, its Z rows (loads issued one block ahead, in zpre) and its
+  // similarity tile (n_sim)
   int n_seg = -1, n_len = 0, n_cell = -1;
   floatx4 n_sim[NT];
-  auto prefetch = [&](int jn) __attribute__((always_inline)) {
-    int lo, hi;
-    wg_range(jn, lo, hi);
-    const int c = lo + wv;
-    n_seg = -1;
-    if (c < hi) {
-      n_seg = a.ch_seg[c];
-      const int p0 = a.ch_start[c];
-      n_len = a.ch_len[c];
-      n_cell = (lane < n_len) ? a.order[p0 + lane] : -1;
-      sim_tile(n_cell, n_len, 0, n_sim);
+  float zpre[NA];
+  auto issue_z = [&](int jn) __attribute__((always_inline)) {
+    const int* e = s_tab + (jn * NW + wv) * kTabStride;
+    load_z(e[4 + lane], e[1], 0, zpre);
+  };
+  auto take_prefetch = [&](int jn) __attribute__((always_inline)) {
+    const int* e = s_tab + (jn * NW + wv) * kTabStride;
+    n_seg = e[0];
+    n_len = e[1];
+    n_cell = e[4 + lane];
+    sim_mfma(zpre, n_sim);
+  };
+  // old-row deltas of a block, loaded one block ahead into registers
+  constexpr int UI = 8;
+  long long vin[UI];
+  const bool in_regs = BK <= UI * kBlock;
+  auto issue_in = [&](int jn) __attribute__((always_inline)) {
+    if (in_regs) {
+#pragma unroll
+      for (int u = 0; u < UI; ++u) vin[u] = a.acc_in[(long long)jn * BK + min(tid + u * kBlock, BK - 1)];
     }
   };
 
@@ -422,20 +522,29 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
     wg_range(j, lo, hi);
     if (lo >= hi) return;
     const int sb = j * B;
-    const int bl = a.ch_seg[lo] - sb, bh = a.ch_seg[hi - 1] - sb;
+    const int bl = s_lohi[4 * j + 2], bh = s_lohi[4 * j + 3];
     const int nq = (bh - bl + 1) * K;
-    unsigned long long sink = 0;
     for (int q = tid; q < nq; q += kBlock) {
       const int p = bl * K + q;
-      sink += fixed_add_wait(&out_rep[(long long)sb * K + p], (double)s_dO[p], a.scale_o);
+      const unsigned long long v =
+          (1ull << kCountShift) + (unsigned long long)llrint((double)s_dO[p] * a.scale_pk);
+      __hip_atomic_fetch_add(reinterpret_cast<unsigned long long*>(&out_rep[(long long)sb * K + p]),
+                             v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    asm volatile("" ::"v"(sink));
   };
 
   // old-row deltas of block jn (complete at launch) and their column sums
   auto load_in = [&](int jn) __attribute__((always_inline)) {
-    for (int q = tid; q < BK; q += kBlock)
-      s_di[q] = (double)a.acc_in[(long long)jn * BK + q] * inv_o;
+    if (in_regs) {
+#pragma unroll
+      for (int u = 0; u < UI; ++u) {
+        const int q = tid + u * kBlock;
+        if (q < BK) s_di[q] = (double)vin[u] * inv_o;
+      }
+    } else {
+      for (int q = tid; q < BK; q += kBlock)
+        s_di[q] = (double)a.acc_in[(long long)jn * BK + q] * inv_o;
+    }
     __syncthreads();
     for (int k = tid; k < K; k += kBlock) {
       double ci = 0.0;
@@ -447,25 +556,69 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
   // O/E update between blocks (clustering.cu:321-347), identical in every
   // workgroup: add the new rows of block jo (jo >= 0), subtract the old rows of
   // the next block (s_di / s_ci); penalty for the next block when there is one
+  // O/E update between blocks (clustering.cu:321-347), identical in every
+  // workgroup: add the new rows of block jo (jo >= 0; read once every
+  // contribution has arrived), subtract the old rows of the next block
+  // (s_di / s_ci); penalty of the next block for the batches this workgroup
+  // processes
   auto update = [&](int jo, bool next) __attribute__((always_inline)) {
     constexpr int U = 4;
-    for (int q0 = tid; q0 < BK; q0 += U * kBlock) {
-      long long vo[U][kSweepRep];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int q = min(q0 + u * kBlock, BK - 1);
-#pragma unroll
-        for (int r = 0; r < kSweepRep; ++r)
-          vo[u][r] = (jo >= 0) ? load_agent(a.acc_out + r * a.rep_stride + (long long)jo * BK + q) : 0;
+    const int jn_pen = (jo >= 0) ? jo + 1 : 0;
+    const int pbl = next ? s_lohi[4 * jn_pen + 2] : 0;
+    const int pbh = next ? s_lohi[4 * jn_pen + 3] : -1;
+    if (jo < 0) {
+      for (int q = tid; q < BK; q += kBlock) s_do[q] = 0.0;
+    } else {
+      const long long* src = a.acc_out + (long long)jo * BK;
+      const int* cnt = s_cnt + jo * B * kSweepRep;
+      // wave 0 waits on one entry per (batch, replica) first, so that the
+      // full read below usually succeeds at once
+      if (wv == 0) {
+        for (int i = lane; i < B * kSweepRep; i += kWave) {
+          const int b = i / kSweepRep, r = i - b * kSweepRep;
+          const long long need = cnt[i];
+          unsigned it = 0;
+          while ((load_agent(src + r * a.rep_stride + b * K) >> kCountShift) < need) {
+            __builtin_amdgcn_s_sleep(1);
+            if (++it >= a.spin_limit) {
+              __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              break;
+            }
+          }
+        }
       }
+      __syncthreads();
+      for (int q0 = tid; q0 < BK; q0 += U * kBlock) {
+        long long vo[U][kSweepRep];
+        unsigned it = 0;
+        while (true) {
+          bool ok = true;
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int q = q0 + u * kBlock;
-        if (q < BK) {
-          long long so = 0;
+          for (int u = 0; u < U; ++u) {
+            const int q = min(q0 + u * kBlock, BK - 1);
+            const int* c = cnt + (q / K) * kSweepRep;
 #pragma unroll
-          for (int r = 0; r < kSweepRep; ++r) so += vo[u][r];
-          s_do[q] = (double)so * inv_o;
+            for (int r = 0; r < kSweepRep; ++r) {
+              vo[u][r] = load_agent(src + r * a.rep_stride + q);
+              ok = ok && ((vo[u][r] >> kCountShift) >= c[r]);
+            }
+          }
+          if (ok) break;
+          __builtin_amdgcn_s_sleep(1);
+          if (++it >= a.spin_limit) {
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int q = q0 + u * kBlock;
+          if (q < BK) {
+            long long so = 0;
+#pragma unroll
+            for (int r = 0; r < kSweepRep; ++r) so += vo[u][r] & kValueMask;
+            s_do[q] = (double)so * a.inv_scale_pk;
+          }
         }
       }
     }
@@ -485,7 +638,8 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
       e = e - pr * (float)s_ci[k];
       s_O[q] = o;
       s_E[q] = e;
-      if (next) s_pen[q] = penalty_value(e, o, a.theta[b], a.stabilized);
+      // the penalty is needed only for the batches this workgroup processes next
+      if (next && b >= pbl && b <= pbh) s_pen[q] = penalty_value(e, o, a.theta[b], a.stabilized);
     }
     __syncthreads();
   };
@@ -495,25 +649,35 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
     for (int k = tid; k < K; k += kBlock) s_ci[k] = 0.0;
   };
 
-  // block 0: remove its old rows, its penalty; first similarity tile
-  __syncthreads();
+  // block 0: remove its old rows, its penalty; first similarity tile; loads
+  // for block 1 in flight
+  issue_in(0);
+  issue_z(0);
   load_in(0);
   update(-1, true);
-  prefetch(0);
+  take_prefetch(0);
+  if (a.nblk > 1) {
+    issue_in(1);
+    issue_z(1);
+  }
 
   for (int j = 0; j < a.nblk; ++j) {
     const bool more = j + 1 < a.nblk;
     rows_block(j);
     flush(j);
-    grid_arrive(a.bar, a.flag, j + 1, G);
-    // overlaps the barrier: next block's old-row deltas and similarity tile
+    // while the other workgroups' sums arrive: the next block's old-row
+    // deltas and similarity tile (both loaded a block ago), then the loads
+    // for the block after it
     if (more) {
       load_in(j + 1);
-      prefetch(j + 1);
+      take_prefetch(j + 1);
+      if (j + 2 < a.nblk) {
+        issue_in(j + 2);
+        issue_z(j + 2);
+      }
     } else {
       clear_in();
     }
-    grid_wait(a.flag, j + 1, a.spin_limit, err);
     update(j, more);
   }
 
@@ -572,8 +736,6 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
       }
       if ((old_s - new_s) < a.tol * fabsf(old_s)) ctrl->done = 1;
     }
-    __hip_atomic_store(a.bar, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(a.flag, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(a.fin, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
