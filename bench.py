@@ -75,7 +75,7 @@ def time_steps(job, first_iter: int, steps: int) -> float:
     return time.perf_counter() - t0
 
 
-def roofline(job, first_iter: int, steps: int, esz: int) -> tuple[dict, dict]:
+def roofline(job, first_iter: int, steps: int, esz: int, precision: str) -> tuple[dict, dict]:
     """Live HIP-event timing of each kernel kind over `steps` Harmony iterations."""
     from rapids_singlecell_amd import _native
 
@@ -115,7 +115,9 @@ def roofline(job, first_iter: int, steps: int, esz: int) -> tuple[dict, dict]:
     tfile = ROOT / "profiles" / "traffic.json"
     if tfile.exists():
         try:
-            traffic = json.loads(tfile.read_text()).get(f"{dom}:{n}x{d}x{K}:{esz}")
+            # HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE passes,
+            # each scaled by what the counter reads for a copy of known size
+            traffic = json.loads(tfile.read_text()).get(f"{dom}:{n}x{d}x{K}:{esz}:{precision}")
         except Exception:
             traffic = None
     roof = {"bound": "hbm", "kernel": f"{dom}_kernel", "achieved": ach, "peak": HBM_PEAK_GBS,
@@ -220,7 +222,7 @@ def main() -> int:
     print(f"[bench] timed {args.steps} steps in {dt:.3f} s", file=sys.stderr, flush=True)
     roof, kernels = (None, None)
     if not args.no_roofline:
-        roof, kernels = roofline(job, it, 2, X.dtype.itemsize)
+        roof, kernels = roofline(job, it, 2, X.dtype.itemsize, args.precision)
         it += 2
     cpu = None
     if not args.no_cpu:

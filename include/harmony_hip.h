@@ -84,7 +84,10 @@ int hz_profile_collect(double* ms_out, long long* launches_out, int n_kinds);
 #define HZ_TUNE_CENTROID_WG 1 /* workgroups of the centroid pass */
 #define HZ_TUNE_SWEEP 2       /* 0 = block sweep in one launch when it fits, 1 = one launch per block */
 #define HZ_TUNE_SWEEP_WG 3    /* workgroups of the block sweep (capped at what is co-resident) */
-#define HZ_TUNE_KEYS 4
+#define HZ_TUNE_SWEEP_PLAIN 4 /* 1 = launch the block sweep as a plain dispatch of the same,
+                                 co-resident size (for profilers that mishandle cooperative
+                                 launches); 0 = cooperative launch */
+#define HZ_TUNE_KEYS 8
 int hz_set_tuning(int key, int value);
 int hz_get_tuning(int key);
 

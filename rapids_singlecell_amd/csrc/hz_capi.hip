@@ -482,6 +482,12 @@ static int launch_sw(const Layout& L, const SweepArgs* a, int* Gp, hipStream_t s
     *Gp = (int)std::max<long long>(1, std::min(G, cap));
     return HZ_OK;
   }
+  if (g_tune[HZ_TUNE_SWEEP_PLAIN]) {
+    // same grid, within the co-resident capacity computed above
+    hipLaunchKernelGGL(kern, dim3((unsigned)*Gp), dim3(kBlock), lds, s, *a);
+    HZ_LAUNCHED("sweep_kernel");
+    return HZ_OK;
+  }
   void* args[] = {const_cast<SweepArgs*>(a)};
   hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kern), dim3((unsigned)*Gp),
                                             dim3(kBlock), args, (unsigned)lds, s);
