@@ -161,6 +161,83 @@ def cpu_baseline(X, codes, B, K, n_sample: int) -> dict:
                       f"(os.cpu_count={os.cpu_count()}, affinity={aff}, blas_threads={blas_threads})"}
 
 
+def served(args) -> int:
+    """Served form: a Poisson stream of integration jobs through HarmonyServer.
+
+    Job shapes: N log-uniform in [5k, 500k], d in {30, 50, 100}, B uniform in
+    [2, 32], theta=2, sigma=0.1, K = min(100, N/30), Harmony defaults (early
+    stopping on), half FP32 and half split-BF16.  Datasets are drawn from a
+    pool generated before the clock starts.  Reports p50/p99 job latency and
+    the aggregate cells x Harmony iterations per second over the run.
+    """
+    import torch
+
+    from rapids_singlecell_amd import _native
+    from rapids_singlecell_amd.datasets import synthetic_pcs
+    from rapids_singlecell_amd.serve import HarmonyServer, latency_summary
+
+    _native.load()
+    rng = np.random.default_rng(args.served_seed)
+    pool = []
+    for i in range(args.served_pool):
+        n = int(np.exp(rng.uniform(np.log(5_000), np.log(500_000))))
+        d = int(rng.choice([30, 50, 100]))
+        B = int(rng.integers(2, 33))
+        X, obs = synthetic_pcs(n, d, int(rng.integers(5, 41)), B, seed=1000 + i)
+        codes = np.asarray(obs["batch"].cat.codes.values, dtype=np.int32)
+        pool.append((X, codes, len(obs["batch"].cat.categories)))
+    print(f"[bench] served: pool of {len(pool)} datasets ready", file=sys.stderr, flush=True)
+    with HarmonyServer(slots=args.served_slots) as srv:
+        # warm-up: one small job per precision
+        for prec in ("native", "split_bf16"):
+            X, c, B = min(pool, key=lambda p: p[0].shape[0])
+            srv.submit(X, c, B, precision=prec, max_iter_harmony=2).result()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        futs, t = [], 0.0
+        while True:
+            t += rng.exponential(1.0 / args.served_rate)
+            if t > args.served_duration:
+                break
+            now = time.perf_counter() - t0
+            if t > now:
+                time.sleep(t - now)
+            X, c, B = pool[int(rng.integers(len(pool)))]
+            prec = "split_bf16" if rng.random() < 0.5 else "native"
+            futs.append((srv.submit(X, c, B, precision=prec, theta=2.0, sigma=0.1),
+                         (X.shape[0], X.shape[1], B, prec)))
+        res, failures = [], []
+        for f, shape in futs:
+            try:
+                res.append(f.result())
+            except Exception as e:  # noqa: BLE001 - reported in the output line
+                failures.append({"shape": shape, "error": str(e)[:200]})
+        wall = time.perf_counter() - t0
+        stats = dict(srv.stats)
+    lats = [info["latency_s"] for _, info in res]
+    cells_iters = sum(info["n_cells"] * info["harmony_iters"] for _, info in res)
+    out = {
+        "metric": METRIC + " (served form)",
+        "value": round(cells_iters / wall, 1),
+        "unit": "cells·iters/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "latency": latency_summary(lats),
+        "jobs": len(res),
+        "offered_rate_jobs_per_s": args.served_rate,
+        "duration_s": args.served_duration,
+        "wall_s": round(wall, 2),
+        "slots": args.served_slots,
+        "server_stats": stats,
+        "failures": failures[:10],
+        "n_failures": len(failures),
+        "data": "synthetic Gaussian-mixture PCA embeddings (pool of "
+                f"{len(pool)} shapes drawn from the served-form distribution)",
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -174,6 +251,13 @@ def main() -> int:
     ap.add_argument("--pmc-run", action="store_true",
                     help="short run for hardware-counter passes: a 256 MiB device copy "
                          "(calibration kernel of known size) then 1 Harmony iteration")
+    ap.add_argument("--served", action="store_true",
+                    help="served form: Poisson job stream, p50/p99 latency")
+    ap.add_argument("--served-rate", type=float, default=20.0)
+    ap.add_argument("--served-duration", type=float, default=30.0)
+    ap.add_argument("--served-slots", type=int, default=4)
+    ap.add_argument("--served-pool", type=int, default=12)
+    ap.add_argument("--served-seed", type=int, default=0)
     ap.add_argument("--tune", action="append", default=[],
                     help="library tuning knob key=value (row_chunk, centroid_wg, no_sweep, sweep_wg)")
     args = ap.parse_args()
@@ -181,6 +265,9 @@ def main() -> int:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the Harmony path "
               "runs on a single MI355X", file=sys.stderr)
         return 2
+
+    if args.served:
+        return served(args)
 
     import torch
 

@@ -58,6 +58,22 @@ int device_cu_count() {
   return g_cus[dev];
 }
 
+// Dynamic LDS of a launch.  The opt-in maximum is set to what the kernel can
+// use at most (device limit minus its static LDS), the same value on every
+// call, so concurrent callers with different sizes never lower it under each
+// other; it is needed once static + dynamic LDS passes the 64 KiB default.
+static int prepare_lds(const void* kern, size_t lds) {
+  hipFuncAttributes fa{};
+  HZ_HIP(hipFuncGetAttributes(&fa, kern));
+  const size_t limit = (size_t)device_lds_limit();
+  const size_t max_dyn = limit > fa.sharedSizeBytes ? limit - fa.sharedSizeBytes : 0;
+  HZ_RET_IF(lds > max_dyn, HZ_ERR_UNSUPPORTED, "kernel needs %zu + %zu bytes of LDS (limit %zu)",
+            lds, (size_t)fa.sharedSizeBytes, limit);
+  if (lds + fa.sharedSizeBytes > 65536)
+    HZ_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_dyn));
+  return HZ_OK;
+}
+
 // ---------------------------------------------------------------- launch timing
 //
 // Opt-in HIP-event timing of the library's own launches (benchmarks only).
@@ -172,6 +188,7 @@ struct Layout {
   Ctrl* ctrl_static;  // never set done: used by the correction pass
   int* counters;      // nblk + 6
   int* swcnt;         // nblk * B * kSweepRep: block-sweep contribution counts
+  int Gs = 1;         // workgroups of the streaming centroid pass
   void* Ypart;
   void* Y;
   void* YT;
@@ -255,6 +272,13 @@ static int make_layout(const hz_dims* dm, void* base, Layout& L) {
     L.cpwm = (int)ceil_div(L.pd.maxch, gm);
     L.Gm = (int)ceil_div(L.pd.maxch, (long long)L.cpwm);
     L.G = std::max(L.G, L.Gm);
+  }
+  if (L.mfma) {
+    // streaming centroid pass (natural cell order): 64-row tiles, grid-stride
+    long long gs = g_tune[HZ_TUNE_CENTROID_WG] > 0 ? g_tune[HZ_TUNE_CENTROID_WG]
+                                                   : 2LL * device_cu_count();
+    L.Gs = (int)std::max<long long>(1, std::min<long long>(gs, ceil_div(L.n, 64)));
+    L.G = std::max(L.G, L.Gs);
   }
   const long long nseg = (long long)L.pd.nblk * L.B;
   const long long BK = (long long)L.B * L.K;
@@ -351,9 +375,7 @@ static size_t row_lds_bytes(const Layout& L) {
 template <typename T, int CT>
 static int launch_row_ct(const RowArgs<T>& a, int grid, size_t lds, hipStream_t s) {
   auto kern = row_kernel<T, CT>;
-  if (lds > 65536)
-    HZ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  HZ_TRY(prepare_lds(reinterpret_cast<const void*>(kern), lds));
   hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), lds, s, a);
   HZ_LAUNCHED("row_kernel");
   return HZ_OK;
@@ -401,9 +423,7 @@ static int launch_centroid(const Layout& L, const CentArgs<T>& a, int grid, hipS
 template <int PREC, int NT, int SM>
 static int launch_rm(RowMArgs a, int grid, size_t lds, hipStream_t s) {
   auto kern = row_mfma_kernel<PREC, NT, SM>;
-  if (lds > 65536)
-    HZ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  HZ_TRY(prepare_lds(reinterpret_cast<const void*>(kern), lds));
   hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), lds, s, a);
   HZ_LAUNCHED("row_mfma_kernel");
   return HZ_OK;
@@ -468,10 +488,7 @@ template <int PREC, int NT, int SM>
 static int launch_sw(const Layout& L, const SweepArgs* a, int* Gp, hipStream_t s) {
   auto kern = sweep_kernel<PREC, NT, SM>;
   const size_t lds = sweep_lds<PREC, SM>(L.Kp, L.B, L.K, L.pd.nblk);
-  if (lds > (size_t)device_lds_limit()) return HZ_ERR_UNSUPPORTED;
-  if (lds > 65536)
-    HZ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  if (prepare_lds(reinterpret_cast<const void*>(kern), lds) != HZ_OK) return HZ_ERR_UNSUPPORTED;
   int per_cu = 0;
   HZ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern),
                                                       kBlock, lds));
@@ -543,20 +560,15 @@ static int launch_sweep(const Layout& L, const SweepArgs* a, int* G, hipStream_t
 
 template <int PREC, int NT, int NU>
 static int launch_cs(const CentSArgs& a, int grid, hipStream_t s) {
-  if constexpr (NT * NU <= 32) {
+  {
     auto kern = centroid_stream_kernel<PREC, NT, NU>;
     const size_t lds = stream_tile_lds<NT, NU>();
     HZ_RET_IF(lds > (size_t)device_lds_limit(), HZ_ERR_UNSUPPORTED,
               "centroid pass needs %zu bytes of LDS", lds);
-    if (lds > 65536)
-      HZ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    HZ_TRY(prepare_lds(reinterpret_cast<const void*>(kern), lds));
     hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), lds, s, a);
     HZ_LAUNCHED("centroid_stream_kernel");
     return HZ_OK;
-  } else {
-    set_error("centroid tile shape %d x %d not instantiated", NT, NU);
-    return HZ_ERR_UNSUPPORTED;
   }
 }
 
@@ -593,9 +605,7 @@ static int launch_cm(const CentMArgs& a, int grid, hipStream_t s) {
     const size_t lds = centroid_tile_lds<PREC, NT, NU>();
     HZ_RET_IF(lds > (size_t)device_lds_limit(), HZ_ERR_UNSUPPORTED,
               "centroid pass needs %zu bytes of LDS", lds);
-    if (lds > 65536)
-      HZ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    HZ_TRY(prepare_lds(reinterpret_cast<const void*>(kern), lds));
     hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), lds, s, a);
     HZ_LAUNCHED("centroid_mfma_kernel");
     return HZ_OK;
@@ -662,13 +672,61 @@ static RowMArgs make_row_margs(const Layout& L, const T* Z, T* R, const EpiArgs<
   return a;
 }
 
-static uint32_t* g_pinned_flag = nullptr;
 
-static int read_done(const Ctrl* ctrl, hipStream_t s, int* done) {
-  if (!g_pinned_flag) HZ_HIP(hipHostMalloc(reinterpret_cast<void**>(&g_pinned_flag), 64));
-  HZ_HIP(hipMemcpyAsync(g_pinned_flag, &ctrl->done, sizeof(int), hipMemcpyDeviceToHost, s));
-  HZ_HIP(hipStreamSynchronize(s));
-  *done = (int)g_pinned_flag[0];
+// Non-blocking convergence checks: the done flag is copied to pinned host
+// memory behind the iterations enqueued so far; the host looks at a copy once
+// its event has completed and blocks only when too many checks are pending.
+// Iterations enqueued after convergence exit at once (every kernel of the
+// loop tests the flag), so over-enqueueing costs launches only.
+struct DoneChecks {
+  static constexpr int kSlots = 8;
+  int* host = nullptr;  // pinned, kSlots ints
+  hipEvent_t ev[kSlots] = {};
+  int head = 0, count = 0;
+};
+static thread_local DoneChecks t_checks[64];
+
+static int checks_init(DoneChecks** out) {
+  DoneChecks& c = t_checks[current_device()];
+  if (!c.host) {
+    HZ_HIP(hipHostMalloc(reinterpret_cast<void**>(&c.host), DoneChecks::kSlots * sizeof(int)));
+    for (auto& e : c.ev) HZ_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  c.head = 0;
+  c.count = 0;
+  *out = &c;
+  return HZ_OK;
+}
+
+static int checks_push(DoneChecks& c, const Ctrl* ctrl, hipStream_t s) {
+  const int slot = (c.head + c.count) % DoneChecks::kSlots;
+  HZ_HIP(hipMemcpyAsync(&c.host[slot], &ctrl->done, sizeof(int), hipMemcpyDeviceToHost, s));
+  HZ_HIP(hipEventRecord(c.ev[slot], s));
+  ++c.count;
+  return HZ_OK;
+}
+
+// Pops completed checks (blocking on the oldest while more than `max_pending`
+// remain); *done = 1 once a copied flag is set.
+static int checks_poll(DoneChecks& c, int max_pending, int* done) {
+  *done = 0;
+  while (c.count > 0) {
+    hipEvent_t e = c.ev[c.head];
+    if (c.count > max_pending) {
+      HZ_HIP(hipEventSynchronize(e));
+    } else {
+      const hipError_t q = hipEventQuery(e);
+      if (q == hipErrorNotReady) break;
+      HZ_HIP(q);
+    }
+    const int v = c.host[c.head];
+    c.head = (c.head + 1) % DoneChecks::kSlots;
+    --c.count;
+    if (v) {
+      *done = 1;
+      return HZ_OK;
+    }
+  }
   return HZ_OK;
 }
 
@@ -878,7 +936,7 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
     // 16-byte loads of R and Z rows in the streaming centroid pass
     const bool aligned = (reinterpret_cast<uintptr_t>(Z) % 16 == 0) &&
                          (reinterpret_cast<uintptr_t>(R) % 16 == 0);
-    use_sw = use_rm && use_cm && aligned && L.K <= 256 && L.pd.nblk <= kSweepMaxBlocks &&
+    use_sw = use_rm && aligned && L.K <= 256 && L.pd.nblk <= kSweepMaxBlocks &&
              g_tune[HZ_TUNE_SWEEP] == 0 &&
              !g_sweep_broken[current_device()];
     sw.Z = Z;
@@ -929,7 +987,7 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
     cs.n = L.n;
     cs.d = L.d;
     cs.K = L.K;
-    gs = (int)std::max<long long>(1, std::min<long long>(L.Gm, ceil_div(L.n, 64)));
+    gs = L.Gs;
     if (use_sw && launch_sweep(L, nullptr, &sweep_grid, s) != HZ_OK) use_sw = false;
   }
 
@@ -969,6 +1027,7 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
   const hipStream_t s2 = ss->s;
   InSumArgs ia{};
   ia.R = reinterpret_cast<const float*>(R);
+  ia.done = &L.ctrl->done;
   ia.acc_in = L.acc_in;
   ia.nseg = (int)nseg;
   ia.K = L.K;
@@ -976,18 +1035,21 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
   auto enqueue_order = [&](int it) -> int {
     ProfScope ps(HZ_PROF_ORDER, s2);
     return launch_block_order(L.pd, seed + (uint32_t)it, cats, L.pb, tabs[it & 1], L.nch_tmp,
-                              nullptr, s2);
+                              &L.ctrl->done, s2);
   };
   // the side stream starts after the work already queued on the caller's stream
   HZ_HIP(hipEventRecord(ss->start, s));
   HZ_HIP(hipStreamWaitEvent(s2, ss->start, 0));
   HZ_TRY(enqueue_order(0));
 
-  const int check_every = 4;
+  DoneChecks* dc = nullptr;
+  HZ_TRY(checks_init(&dc));
+  const int check_every = 2;
   for (int it = 0; it < max_iter; ++it) {
     if (it > 0 && it % check_every == 0) {
+      HZ_TRY(checks_push(*dc, L.ctrl, s));
       int done = 0;
-      HZ_TRY(read_done(L.ctrl, s, &done));
+      HZ_TRY(checks_poll(*dc, 3, &done));
       if (done) break;
     }
     // side stream: the old-row sums of this iteration's blocks (once R of the
@@ -1004,8 +1066,8 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
       HZ_TRY(fill<long long>(L.acc_in, nseg * L.K, 0, s2));
       HZ_TRY(launch_insum(L, ia, s2));
       hipLaunchKernelGGL(sweep_count_kernel, dim3(L.pd.nblk), dim3(kBlock),
-                         (size_t)L.B * kSweepRep * sizeof(int), s2, t.seg_ch, t.ch_seg, L.swcnt,
-                         L.B, sweep_grid);
+                         (size_t)L.B * kSweepRep * sizeof(int), s2, &L.ctrl->done, t.seg_ch,
+                         t.ch_seg, L.swcnt, L.B, sweep_grid);
       HZ_LAUNCHED("sweep_count_kernel");
     }
     HZ_HIP(hipEventRecord(ss->ready[it & 1], s2));
@@ -1092,9 +1154,7 @@ template <typename T, int DT>
 static int launch_apply_dt(const Layout& L, const ApplyArgs<T>& a, int grid, size_t lds,
                            hipStream_t s) {
   auto kern = apply_kernel<T, DT>;
-  if (lds > 65536)
-    HZ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  HZ_TRY(prepare_lds(reinterpret_cast<const void*>(kern), lds));
   hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), lds, s, a);
   HZ_LAUNCHED("apply_kernel");
   return HZ_OK;
@@ -1169,9 +1229,7 @@ static int correction_impl(const hz_dims* dm, const T* X, const T* R, const T* O
       lds += (size_t)((L.B + 1) * (L.B + 1) + (L.B + 1) * L.d) * sizeof(double);
     HZ_RET_IF(lds > (size_t)device_lds_limit(), HZ_ERR_UNSUPPORTED,
               "correction solve needs %zu bytes of LDS", lds);
-    if (lds > 65536)
-      HZ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(w_kernel<T>),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    HZ_TRY(prepare_lds(reinterpret_cast<const void*>(w_kernel<T>), lds));
     hipLaunchKernelGGL(w_kernel<T>, dim3(L.K), dim3(kBlock), lds, s, wa);
     HZ_LAUNCHED("w_kernel");
   }
@@ -1359,8 +1417,7 @@ static int kmeans_impl(const T* Z, long long n, int d, int k, uint32_t seed, int
       case 8: fn = reinterpret_cast<const void*>(kmeans_assign_kernel<T, 8>); break;
       default: fn = reinterpret_cast<const void*>(kmeans_assign_kernel<T, 16>); break;
     }
-    if (lds > 65536)
-      HZ_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    HZ_TRY(prepare_lds(fn, lds));
     switch (L.CT) {
       case 4: hipLaunchKernelGGL((kmeans_assign_kernel<T, 4>), dim3(L.G), dim3(kBlock), lds, s, a); break;
       case 8: hipLaunchKernelGGL((kmeans_assign_kernel<T, 8>), dim3(L.G), dim3(kBlock), lds, s, a); break;

@@ -49,6 +49,7 @@ struct InSumArgs {
   const int* ch_len;
   const int* ch_seg;
   long long* acc_in;  // nseg x K, zero at launch
+  const int* done;    // convergence flag: nothing to do once set
   int nseg, K;
   double scale_o;
 };
@@ -64,6 +65,7 @@ __global__ __launch_bounds__(kBlock) void insum_kernel(InSumArgs a) {
   __shared__ float s_part[NW][kWave * KW];
   __shared__ int s_seg[NW];
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
+  if (*a.done) return;
   const int K = a.K;
   const int nch = a.seg_ch[a.nseg];
   const int c = blockIdx.x * NW + wv;
@@ -121,10 +123,12 @@ __global__ __launch_bounds__(kBlock) void insum_kernel(InSumArgs a) {
 // [nlo + w*per, min(nhi, nlo + (w+1)*per)) of block j and adds to every batch
 // from its first chunk's to its last chunk's, in replica w % kSweepRep (the
 // same rule as the sweep's flush).  One workgroup per block.
-__global__ __launch_bounds__(kBlock) void sweep_count_kernel(const int* __restrict__ seg_ch,
+__global__ __launch_bounds__(kBlock) void sweep_count_kernel(const int* __restrict__ done,
+                                                             const int* __restrict__ seg_ch,
                                                              const int* __restrict__ ch_seg,
                                                              int* __restrict__ cnt, int B, int G) {
   extern __shared__ int s_c[];  // B x kSweepRep
+  if (*done) return;
   const int j = blockIdx.x;
   for (int i = threadIdx.x; i < B * kSweepRep; i += blockDim.x) s_c[i] = 0;
   __syncthreads();

@@ -324,3 +324,24 @@ def test_clustering_sweep_vs_per_block(device, tuning, precision, stabilized):
         assert rel(E, Er) < 1e-4
         np.testing.assert_allclose(last, last_r, rtol=1e-4)
     assert rel(out[0][0], out[1][0]) < 1e-4
+
+
+@pytest.mark.parametrize("precision", ["native", "split_bf16"])
+@pytest.mark.parametrize("n,d,B", [(5395, 30, 7), (5836, 100, 25), (11227, 30, 28), (29263, 50, 32)])
+def test_harmonize_shapes(device, precision, n, d, B):
+    """Default-K jobs of the served-form distribution (LDS sizes around the
+    64 KiB opt-in boundary, d=100, up to 32 batches) run end to end and agree
+    with the FP64 oracle at fixed iteration counts."""
+    X, obs = synthetic_pcs(n, d, 12, B, seed=n % 97)
+    X = X.astype(np.float32)
+    codes, cats, _ = ho.batch_codes(obs, "batch")
+    K = max(int(min(100, n / 30)), 2)
+    Z64 = ho.l2_normalize(X.astype(np.float64))
+    Y0 = Z64[np.random.default_rng(5).choice(n, K, replace=False)]
+    kw = dict(n_clusters=K, max_iter_harmony=2, max_iter_clustering=3, random_state=2,
+              tol_harmony=float("-inf"), tol_clustering=float("-inf"))
+    Zg = harmonize(_t(X, device), obs, "batch", init_centroids=Y0.astype(np.float32),
+                   precision=precision, **kw).cpu().numpy()
+    Zr = ho.harmonize(X.astype(np.float64), codes, len(cats), Y0=Y0, **kw)
+    assert np.all(np.isfinite(Zg))
+    assert rel(Zg, Zr) < 1e-3, rel(Zg, Zr)
