@@ -191,13 +191,17 @@ constexpr long long kValueMask = (1LL << kCountShift) - 1;
 // the next block can be fetched without a chain of dependent loads).
 constexpr int kTabStride = 68;  // ints per (block, wave) entry: seg, len, p0, pad, 64 cells
 constexpr int kSweepMaxBlocks = 64;
+// rounds (of NW chunks per workgroup) of one block whose chunks are staged in
+// the round table one block ahead; later rounds fetch their chunk inline
+constexpr int kRoundTab = 8;
 
 template <int PREC, int SM>
 __host__ __device__ inline size_t sweep_lds(int Kp, int B, int K, int nblk) {
   const size_t y = align_up(row_mfma_lds<PREC, SM>(Kp), 16);
   return y + (size_t)3 * B * K * 4 + (size_t)2 * B * K * 8 + (size_t)2 * K * 8 +
          (size_t)nblk * (kBlock / kWave) * kTabStride * 4 + (size_t)nblk * 4 * 4 +
-         (size_t)nblk * B * kSweepRep * 4 + 64;
+         (size_t)nblk * B * kSweepRep * 4 + (size_t)kRoundTab * (kBlock / kWave) * kTabStride * 4 +
+         64;
 }
 
 template <int PREC, int NT, int SM>
@@ -233,6 +237,7 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
   int* s_tab = reinterpret_cast<int*>(s_ci + K);  // nblk x NW x kTabStride
   int* s_lohi = s_tab + a.nblk * NW * kTabStride; // nblk x 4: chunk range, batch range
   int* s_cnt = s_lohi + 4 * a.nblk;  // nblk x B x kSweepRep: contributions per acc_out entry
+  int* s_rtab = s_cnt + a.nblk * B * kSweepRep;  // kRoundTab x NW x kTabStride: next block's rounds
 
   // stage the normalised centroids (same padded layouts as the row kernel)
   const int KpS = Kp + 16;
@@ -322,7 +327,11 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
   __syncthreads();
 
   // similarities of 16 rows (r0 .. r0+15 of a chunk) on matrix cores
-  // A operand of one 16-row tile: row r0 + lr of the chunk (zero beyond len)
+  // A operand of one 16-row tile: row r0 + lr of the chunk.  Nothing is
+  // masked after the load, so the loads stay in flight until the MFMAs use
+  // them: a row past len reads row 0 (its results are never stored) and a
+  // column c >= d reads column d-1, which meets the zero rows c >= d of the
+  // staged centroids.
   auto load_z = [&](int my_cell, int len, int r0, float (&za)[NA]) __attribute__((always_inline)) {
     const int zc = __shfl(my_cell, (r0 + lr) & 63, kWave);
     const bool zvalid = r0 + lr < len;
@@ -330,8 +339,7 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
 #pragma unroll
     for (int i = 0; i < NA; ++i) {
       const int cc = (PREC == kPrecF32) ? 4 * i + lg : 32 * (i / 8) + 8 * lg + (i % 8);
-      const float zv = a.Z[zbase + (unsigned)min(cc, d - 1)];
-      za[i] = zv * ((zvalid && cc < d) ? 1.f : 0.f);
+      za[i] = a.Z[zbase + (unsigned)min(cc, d - 1)];
     }
   };
   // MFMA chain of one tile; the B fragments (centroids in LDS) of the next
@@ -463,38 +471,104 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
     }
   };
 
-  // rows of block j for this workgroup (new R; per-block sums into s_dO)
+  // chunks of rounds 1 .. kRoundTab-1 of block jn (round 0 is in s_tab):
+  // entry (r, wave) holds segment, length, start and the 64 cell ids of the
+  // wave's chunk lo + r*NW + wave.  Two rounds of independent loads, issued
+  // while the other workgroups' sums of the current block arrive.
+  auto stage_rounds = [&](int jn) __attribute__((always_inline)) {
+    int lo, hi;
+    wg_range(jn, lo, hi);
+    if (lo + NW >= hi) return;  // a single round
+    if (lane < kRoundTab && lane >= 1) {
+      const int c = lo + lane * NW + wv;
+      int* e = s_rtab + (lane * NW + wv) * kTabStride;
+      if (c < hi) {
+        e[0] = a.ch_seg[c];
+        e[1] = a.ch_len[c];
+        e[2] = a.ch_start[c];
+      } else {
+        e[0] = -1;
+        e[1] = 0;
+        e[2] = 0;
+      }
+    }
+    __syncthreads();
+    int cl[kRoundTab];
+#pragma unroll
+    for (int r = 1; r < kRoundTab; ++r) {
+      const int* e = s_rtab + (r * NW + wv) * kTabStride;
+      const int len = e[1];
+      const int x = a.order[(lane < len) ? e[2] + lane : 0];  // order[0] is always valid
+      cl[r] = (lane < len) ? x : -1;
+    }
+#pragma unroll
+    for (int r = 1; r < kRoundTab; ++r) s_rtab[(r * NW + wv) * kTabStride + 4 + lane] = cl[r];
+    __syncthreads();
+  };
+
+  // rows of block j for this workgroup (new R; per-block sums into s_dO).
+  // Round 0 uses the prefetched tile; the Z rows of round r+1 are loaded
+  // while round r is normalised.
   auto rows_block = [&](int j) __attribute__((always_inline)) {
     int lo, hi;
     wg_range(j, lo, hi);
     const int sb = j * B;
     for (int q = tid; q < BK; q += kBlock) s_dO[q] = 0.f;
-    for (int cb = lo; cb < hi; cb += NW) {
+    float zn[NA], zc[NA];
+    int cseg = -1, clen = 0, ccell = -1;
+    auto round_chunk = [&](int r, int& seg, int& len, int& cell) __attribute__((always_inline)) {
+      const int c = lo + r * NW + wv;
+      if (r < kRoundTab) {
+        const int* e = s_rtab + (r * NW + wv) * kTabStride;
+        seg = e[0];
+        len = e[1];
+        cell = e[4 + lane];
+      } else {
+        seg = (c < hi) ? a.ch_seg[c] : -1;
+        len = (c < hi) ? a.ch_len[c] : 0;
+        const int p0 = (c < hi) ? a.ch_start[c] : 0;
+        const int x = a.order[(lane < len) ? p0 + lane : 0];
+        cell = (lane < len) ? x : -1;
+      }
+    };
+    int r = 0;
+    for (int cb = lo; cb < hi; cb += NW, ++r) {
       const int c = cb + wv;
       int seg = -1;
+      // next round's chunk and Z rows (loads in flight during this round)
+      const bool next = cb + NW < hi;
+      int nseg2 = -1, nlen2 = 0, ncell2 = -1;
+      if (next) {
+        round_chunk(r + 1, nseg2, nlen2, ncell2);
+        if (cb + NW + wv < hi) load_z(ncell2, nlen2, 0, zn);
+      }
       if (c < hi) {
         float dO[NT];
 #pragma unroll
         for (int t = 0; t < NT; ++t) dO[t] = 0.f;
         int len, cell;
-        if (cb == lo) {
+        floatx4 acc[NT];
+        if (r == 0) {
           seg = n_seg;
           len = n_len;
           cell = n_cell;
         } else {
-          seg = a.ch_seg[c];
-          const int p0 = a.ch_start[c];
-          len = a.ch_len[c];
-          cell = (lane < len) ? a.order[p0 + lane] : -1;
+          seg = cseg;
+          len = clen;
+          cell = ccell;
+          sim_mfma(zc, acc);
         }
         const int b = seg - sb;
         for (int r0 = 0; r0 < len; r0 += 16) {
-          if (cb == lo && r0 == 0) {
-            finish_tile(b, cell, len, 0, n_sim, dO);
+          if (r0 == 0) {
+            if (r == 0)
+              finish_tile(b, cell, len, 0, n_sim, dO);
+            else
+              finish_tile(b, cell, len, 0, acc, dO);
           } else {
-            floatx4 acc[NT];
-            sim_tile(cell, len, r0, acc);
-            finish_tile(b, cell, len, r0, acc, dO);
+            floatx4 acc2[NT];
+            sim_tile(cell, len, r0, acc2);
+            finish_tile(b, cell, len, r0, acc2, dO);
           }
         }
 #pragma unroll
@@ -505,6 +579,13 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
           const int k = 16 * t + lr;
           if (lg == 0 && k < K) s_part[wv][k] = x;
         }
+      }
+      if (next) {
+        cseg = nseg2;
+        clen = nlen2;
+        ccell = ncell2;
+#pragma unroll
+        for (int i = 0; i < NA; ++i) zc[i] = zn[i];
       }
       if (lane == 0) s_seg[wv] = seg;
       __syncthreads();
@@ -660,6 +741,7 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
   load_in(0);
   update(-1, true);
   take_prefetch(0);
+  stage_rounds(0);
   if (a.nblk > 1) {
     issue_in(1);
     issue_z(1);
@@ -675,6 +757,7 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
     if (more) {
       load_in(j + 1);
       take_prefetch(j + 1);
+      stage_rounds(j + 1);
       if (j + 2 < a.nblk) {
         issue_in(j + 2);
         issue_z(j + 2);
