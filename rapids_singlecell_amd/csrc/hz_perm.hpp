@@ -15,7 +15,11 @@
 
 namespace hz {
 
-constexpr int kScanThreads = 1024;
+// Scans run in one workgroup of four waves with few registers, so that they
+// fit on a CU beside a resident clustering sweep (a larger workgroup would
+// wait for the sweep to end).
+constexpr int kScanThreads = 256;
+constexpr int kScanPer = 8;  // consecutive elements per thread and tile
 constexpr int kGroupTile = 1024;  // positions per tile in the grouping passes
 
 __device__ __forceinline__ uint32_t pcg_key(uint32_t i, uint32_t seed) {
@@ -114,33 +118,53 @@ __global__ void perm_keys_kernel(const int* __restrict__ done, int n, uint32_t s
   }
 }
 
-// Exclusive scan of `n` ints by one workgroup of kScanThreads threads.
+// Exclusive scan of `n` ints by one workgroup of kScanThreads threads, in
+// coalesced tiles of kScanThreads * kScanPer elements with a running carry.
 // out[0..n] (out[n] = total).  Optional second output `out2[0..n-1]`.
+// `lds` holds at least kScanThreads / 64 ints.
 __device__ void wg_exclusive_scan(const int* __restrict__ in, int* __restrict__ out,
                                   int* __restrict__ out2, long long n, int* lds) {
-  const int t = threadIdx.x;
-  const long long per = (n + kScanThreads - 1) / kScanThreads;
-  const long long lo = std::min<long long>(n, (long long)t * per);
-  const long long hi = std::min<long long>(n, lo + per);
-  int s = 0;
-  for (long long i = lo; i < hi; ++i) s += in[i];
-  lds[t] = s;
-  __syncthreads();
-  // Hillis-Steele inclusive scan over the per-thread sums.
-  for (int off = 1; off < kScanThreads; off <<= 1) {
-    int v = (t >= off) ? lds[t - off] : 0;
+  constexpr int kWaves = kScanThreads / 64;
+  constexpr int kTile = kScanThreads * kScanPer;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  int carry = 0;
+  for (long long base = 0; base < n; base += kTile) {
+    const long long i0 = base + (long long)t * kScanPer;
+    int v[kScanPer];
+    int sum = 0;
+#pragma unroll
+    for (int u = 0; u < kScanPer; ++u) {
+      v[u] = (i0 + u < n) ? in[i0 + u] : 0;
+      sum += v[u];
+    }
+    int x = sum;  // inclusive scan of the thread sums across the wave
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int y = __shfl_up(x, off, 64);
+      if (lane >= off) x += y;
+    }
+    if (lane == 63) lds[w] = x;
     __syncthreads();
-    lds[t] += v;
+    int wpre = 0, tot = 0;
+#pragma unroll
+    for (int q = 0; q < kWaves; ++q) {
+      const int wt = lds[q];
+      wpre += (q < w) ? wt : 0;
+      tot += wt;
+    }
     __syncthreads();
+    int run = carry + wpre + x - sum;
+#pragma unroll
+    for (int u = 0; u < kScanPer; ++u) {
+      if (i0 + u < n) {
+        out[i0 + u] = run;
+        if (out2) out2[i0 + u] = run;
+      }
+      run += v[u];
+    }
+    carry += tot;
   }
-  int run = (t == 0) ? 0 : lds[t - 1];
-  for (long long i = lo; i < hi; ++i) {
-    int v = in[i];
-    out[i] = run;
-    if (out2) out2[i] = run;
-    run += v;
-  }
-  if (t == kScanThreads - 1) out[n] = lds[kScanThreads - 1];
+  if (t == 0) out[n] = carry;
   __syncthreads();
 }
 
@@ -165,10 +189,14 @@ __global__ void perm_scatter_kernel(const int* __restrict__ done, int n, int shi
 }
 
 // Exact rank inside each bucket: keys are distinct, so position = bucket start
-// + number of smaller keys in the bucket.
+// + number of smaller keys in the bucket.  With `thist`, each cell is also
+// counted in the (block, batch, tile) histogram of its position (tiles of
+// kGroupTile positions; `thist` zeroed beforehand).
 __global__ void perm_rank_kernel(const int* __restrict__ done, int n, int shift,
                                  const uint32_t* __restrict__ tk, const int* __restrict__ ti,
-                                 const int* __restrict__ bstart, int* __restrict__ perm) {
+                                 const int* __restrict__ bstart, int* __restrict__ perm,
+                                 const int* __restrict__ cats, int bs, int nb, int tpb,
+                                 int* __restrict__ thist) {
   if (done && *done) return;
   for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
     uint32_t k = tk[s];
@@ -176,26 +204,15 @@ __global__ void perm_rank_kernel(const int* __restrict__ done, int n, int shift,
     int lo = bstart[b], hi = bstart[b + 1];
     int r = 0;
     for (int t = lo; t < hi; ++t) r += (tk[t] < k) ? 1 : 0;
-    perm[lo + r] = ti[s];
+    const int pos = lo + r;
+    const int cell = ti[s];
+    perm[pos] = cell;
+    if (thist) {
+      const int j = pos / bs;
+      const int tile = (pos - j * bs) / kGroupTile;
+      atomicAdd(&thist[((long long)j * nb + cats[cell]) * tpb + tile], 1);
+    }
   }
-}
-
-// Count batches per (block, tile) of the key-sorted order.  One wave per tile.
-__global__ __launch_bounds__(64) void group_count_kernel(
-    const int* __restrict__ done, const int* __restrict__ perm, const int* __restrict__ cats,
-    int n, int bs, int nb, int tpb, int* __restrict__ thist) {
-  extern __shared__ int cnt[];
-  if (done && *done) return;
-  const int j = blockIdx.x / tpb, t = blockIdx.x % tpb;
-  const long long blk_end = std::min<long long>((long long)(j + 1) * bs, n);
-  const long long p0 = (long long)j * bs + (long long)t * kGroupTile;
-  const long long p1 = std::min<long long>(p0 + kGroupTile, blk_end);
-  for (int b = threadIdx.x; b < nb; b += blockDim.x) cnt[b] = 0;
-  __syncthreads();
-  for (long long p = p0 + threadIdx.x; p < p1; p += blockDim.x) atomicAdd(&cnt[cats[perm[p]]], 1);
-  __syncthreads();
-  for (int b = threadIdx.x; b < nb; b += blockDim.x)
-    thist[((long long)j * nb + b) * tpb + t] = cnt[b];
 }
 
 // Stable placement: cells of tile (block j, tile t) go to their batch segment
@@ -211,9 +228,31 @@ __global__ __launch_bounds__(64) void group_place_kernel(
   const long long p0 = (long long)j * bs + (long long)t * kGroupTile;
   const long long p1 = std::min<long long>(p0 + kGroupTile, blk_end);
   const int lane = threadIdx.x;
+  const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  if (nb <= 64) {
+    // lane b holds the next free slot of batch b in this tile: no LDS and no
+    // barriers, one ballot round per distinct batch of each 64 positions
+    int slot = (lane < nb) ? tbase[((long long)j * nb + lane) * tpb + t] : 0;
+    for (long long base = p0; base < p1; base += 64) {
+      const long long p = base + lane;
+      const bool valid = p < p1;
+      const int cell = valid ? perm[p] : -1;
+      const int b = valid ? cats[cell] : -1;
+      unsigned long long active = __ballot(valid);
+      while (active) {
+        const int leader = __builtin_ctzll(active);
+        const int b0 = __builtin_amdgcn_readlane(b, leader);
+        const unsigned long long m = __ballot(b == b0) & active;
+        const int start = __builtin_amdgcn_readlane(slot, b0);
+        if (b == b0) order[start + __popcll(m & lt)] = cell;
+        if (lane == b0) slot += __popcll(m);
+        active &= ~m;
+      }
+    }
+    return;
+  }
   for (int b = lane; b < nb; b += 64) cnt[b] = 0;
   __syncthreads();
-  const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   for (long long base = p0; base < p1; base += 64) {
     long long p = base + lane;
     bool valid = p < p1;
@@ -237,7 +276,8 @@ __global__ __launch_bounds__(64) void group_place_kernel(
   }
 }
 
-// Segment starts and the chunk table from per-tile offsets (one workgroup).
+// Segment starts and first chunk of each segment from per-tile offsets (one
+// workgroup; chunk_fill_kernel then writes the chunk entries).
 // `tpb` = tiles per segment (1 for a plain segment-offset input).
 __global__ __launch_bounds__(kScanThreads) void chunk_table_kernel(
     const int* __restrict__ done, const int* __restrict__ thist, int* __restrict__ tbase,
@@ -259,16 +299,29 @@ __global__ __launch_bounds__(kScanThreads) void chunk_table_kernel(
   }
   __syncthreads();
   wg_exclusive_scan(nch_tmp, seg_ch, nullptr, nseg, lds);
-  for (long long s = threadIdx.x; s < nseg; s += blockDim.x) {
-    int c0 = seg_ch[s];
-    int st = seg_start[s];
-    int len = seg_start[s + 1] - st;
-    int m = 0;
-    for (int off = 0; off < len; off += chunk, ++m) {
-      ch_start[c0 + m] = st + off;
-      ch_len[c0 + m] = std::min(chunk, len - off);
-      ch_seg[c0 + m] = (int)s;
+}
+
+// One thread per chunk: its segment is the last s with seg_ch[s] <= c (empty
+// segments share their seg_ch with the next one), then start, length, segment.
+__global__ void chunk_fill_kernel(const int* __restrict__ done, long long nseg, int chunk,
+                                  const int* __restrict__ seg_start,
+                                  const int* __restrict__ seg_ch, int* __restrict__ ch_start,
+                                  int* __restrict__ ch_len, int* __restrict__ ch_seg,
+                                  long long maxch) {
+  if (done && *done) return;
+  const long long total = seg_ch[nseg];
+  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < total && c < maxch;
+       c += (long long)gridDim.x * blockDim.x) {
+    long long lo = 0, hi = nseg - 1;  // seg_ch[0] = 0 <= c
+    while (lo < hi) {
+      const long long mid = (lo + hi + 1) / 2;
+      if (seg_ch[mid] <= c) lo = mid;
+      else hi = mid - 1;
     }
+    const int st = seg_start[lo] + (int)(c - seg_ch[lo]) * chunk;
+    ch_start[c] = st;
+    ch_len[c] = min(chunk, seg_start[lo + 1] - st);
+    ch_seg[c] = (int)lo;
   }
 }
 
@@ -279,9 +332,10 @@ static inline int launch_grid_1d(long long n, int per_cu = 8) {
   return (int)std::max<long long>(1, std::min<long long>(g, (long long)device_cu_count() * per_cu));
 }
 
-// Key-sorted order for `seed` into b.perm (4 launches + 1 fill).
+// Key-sorted order for `seed` into b.perm (4 launches + 1 fill).  With
+// `cats`, the rank pass also fills the (block, batch, tile) histogram b.thist.
 static int launch_key_order(const PermDims& p, uint32_t seed, PermBufs& b, const int* done,
-                            hipStream_t s) {
+                            hipStream_t s, const int* cats = nullptr) {
   const int shift = 32 - p.nbk_bits;
   const int nbk = 1 << p.nbk_bits;
   HZ_TRY(fill<int>(b.bhist, nbk, 0, s));
@@ -295,8 +349,9 @@ static int launch_key_order(const PermDims& p, uint32_t seed, PermBufs& b, const
   hipLaunchKernelGGL(perm_scatter_kernel, dim3(g), dim3(256), 0, s, done, p.n, shift, b.keys,
                      b.bcur, b.tk, b.ti);
   HZ_LAUNCHED("perm_scatter");
+  if (cats) HZ_TRY(fill<int>(b.thist, (long long)p.nblk * p.nb * p.tpb, 0, s));
   hipLaunchKernelGGL(perm_rank_kernel, dim3(g), dim3(256), 0, s, done, p.n, shift, b.tk, b.ti,
-                     b.bstart, b.perm);
+                     b.bstart, b.perm, cats, p.bs, p.nb, p.tpb, cats ? b.thist : nullptr);
   HZ_LAUNCHED("perm_rank");
   return HZ_OK;
 }
@@ -304,17 +359,18 @@ static int launch_key_order(const PermDims& p, uint32_t seed, PermBufs& b, const
 // Full block order + grouping + chunk table for one clustering iteration.
 static int launch_block_order(const PermDims& p, uint32_t seed, const int* cats, PermBufs& b,
                               ChunkTab& t, int* nch_tmp, const int* done, hipStream_t s) {
-  HZ_TRY(launch_key_order(p, seed, b, done, s));
+  HZ_TRY(launch_key_order(p, seed, b, done, s, cats));
   const long long nseg = (long long)p.nblk * p.nb;
   const size_t lds = (size_t)p.nb * sizeof(int);
   const int tiles = p.nblk * p.tpb;
-  hipLaunchKernelGGL(group_count_kernel, dim3(tiles), dim3(64), lds, s, done, b.perm, cats, p.n,
-                     p.bs, p.nb, p.tpb, b.thist);
-  HZ_LAUNCHED("group_count");
   hipLaunchKernelGGL(chunk_table_kernel, dim3(1), dim3(kScanThreads), 0, s, done, b.thist,
                      b.tbase, nseg, p.tpb, p.n, p.chunk, t.seg_start, t.seg_ch, t.ch_start,
                      t.ch_len, t.ch_seg, nch_tmp);
   HZ_LAUNCHED("chunk_table");
+  hipLaunchKernelGGL(chunk_fill_kernel, dim3(launch_grid_1d(p.maxch)), dim3(256), 0, s, done,
+                     nseg, p.chunk, t.seg_start, t.seg_ch, t.ch_start, t.ch_len, t.ch_seg,
+                     p.maxch);
+  HZ_LAUNCHED("chunk_fill");
   hipLaunchKernelGGL(group_place_kernel, dim3(tiles), dim3(64), lds, s, done, b.perm, cats, p.n,
                      p.bs, p.nb, p.tpb, b.tbase, t.order);
   HZ_LAUNCHED("group_place");
@@ -342,6 +398,11 @@ static int launch_static_chunks(int n, int nb, int chunk, const int* cat_offsets
                      nullptr, (long long)nb, 1, n, chunk, t.seg_start, t.seg_ch, t.ch_start,
                      t.ch_len, t.ch_seg, nch_tmp);
   HZ_LAUNCHED("chunk_table_static");
+  const long long maxch = ceil_div(n, chunk) + nb;
+  hipLaunchKernelGGL(chunk_fill_kernel, dim3(launch_grid_1d(maxch)), dim3(256), 0, s, nullptr,
+                     (long long)nb, chunk, t.seg_start, t.seg_ch, t.ch_start, t.ch_len, t.ch_seg,
+                     maxch);
+  HZ_LAUNCHED("chunk_fill_static");
   return HZ_OK;
 }
 
