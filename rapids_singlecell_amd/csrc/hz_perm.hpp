@@ -189,21 +189,40 @@ __global__ void perm_scatter_kernel(const int* __restrict__ done, int n, int shi
 }
 
 // Exact rank inside each bucket: keys are distinct, so position = bucket start
-// + number of smaller keys in the bucket.  With `thist`, each cell is also
-// counted in the (block, batch, tile) histogram of its position (tiles of
-// kGroupTile positions; `thist` zeroed beforehand).
-__global__ void perm_rank_kernel(const int* __restrict__ done, int n, int shift,
-                                 const uint32_t* __restrict__ tk, const int* __restrict__ ti,
-                                 const int* __restrict__ bstart, int* __restrict__ perm,
-                                 const int* __restrict__ cats, int bs, int nb, int tpb,
-                                 int* __restrict__ thist) {
+// + number of smaller keys in the bucket.  A workgroup ranks kRankTile
+// consecutive slots of the bucketed keys; the keys of every bucket those slots
+// touch are staged in LDS once (up to kRankLds of them, else read from
+// memory).  With `thist`, each cell is also counted in the (block, batch,
+// tile) histogram of its position (tiles of kGroupTile positions; `thist`
+// zeroed beforehand).
+constexpr int kRankTile = 512;
+constexpr int kRankLds = 2048;
+__global__ __launch_bounds__(256) void perm_rank_kernel(
+    const int* __restrict__ done, int n, int shift, const uint32_t* __restrict__ tk,
+    const int* __restrict__ ti, const int* __restrict__ bstart, int* __restrict__ perm,
+    const int* __restrict__ cats, int bs, int nb, int tpb, int* __restrict__ thist) {
+  __shared__ uint32_t sk[kRankLds];
   if (done && *done) return;
-  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
-    uint32_t k = tk[s];
-    int b = (int)(k >> shift);
-    int lo = bstart[b], hi = bstart[b + 1];
+  const int s0 = blockIdx.x * kRankTile;
+  const int s1 = min(n, s0 + kRankTile);
+  // buckets are in slot order: the tile's keys lie in [L0, L1)
+  const int L0 = bstart[(int)(tk[s0] >> shift)];
+  const int L1 = bstart[(int)(tk[s1 - 1] >> shift) + 1];
+  const bool staged = L1 - L0 <= kRankLds;
+  if (staged) {
+    for (int t = L0 + (int)threadIdx.x; t < L1; t += blockDim.x) sk[t - L0] = tk[t];
+    __syncthreads();
+  }
+  for (int s = s0 + (int)threadIdx.x; s < s1; s += blockDim.x) {
+    const uint32_t k = tk[s];
+    const int b = (int)(k >> shift);
+    const int lo = bstart[b], hi = bstart[b + 1];
     int r = 0;
-    for (int t = lo; t < hi; ++t) r += (tk[t] < k) ? 1 : 0;
+    if (staged) {
+      for (int t = lo - L0; t < hi - L0; ++t) r += (sk[t] < k) ? 1 : 0;
+    } else {
+      for (int t = lo; t < hi; ++t) r += (tk[t] < k) ? 1 : 0;
+    }
     const int pos = lo + r;
     const int cell = ti[s];
     perm[pos] = cell;
@@ -350,7 +369,8 @@ static int launch_key_order(const PermDims& p, uint32_t seed, PermBufs& b, const
                      b.bcur, b.tk, b.ti);
   HZ_LAUNCHED("perm_scatter");
   if (cats) HZ_TRY(fill<int>(b.thist, (long long)p.nblk * p.nb * p.tpb, 0, s));
-  hipLaunchKernelGGL(perm_rank_kernel, dim3(g), dim3(256), 0, s, done, p.n, shift, b.tk, b.ti,
+  hipLaunchKernelGGL(perm_rank_kernel, dim3((int)ceil_div(p.n, kRankTile)), dim3(256), 0, s,
+                     done, p.n, shift, b.tk, b.ti,
                      b.bstart, b.perm, cats, p.bs, p.nb, p.tpb, cats ? b.thist : nullptr);
   HZ_LAUNCHED("perm_rank");
   return HZ_OK;
