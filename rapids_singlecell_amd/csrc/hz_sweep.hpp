@@ -201,7 +201,7 @@ __host__ __device__ inline size_t sweep_lds(int Kp, int B, int K, int nblk) {
   return y + (size_t)3 * B * K * 4 + (size_t)2 * B * K * 8 + (size_t)2 * K * 8 +
          (size_t)nblk * (kBlock / kWave) * kTabStride * 4 + (size_t)nblk * 4 * 4 +
          (size_t)nblk * B * kSweepRep * 4 + (size_t)kRoundTab * (kBlock / kWave) * kTabStride * 4 +
-         64;
+         (size_t)2 * B * 4 + 64;
 }
 
 template <int PREC, int NT, int SM>
@@ -238,6 +238,8 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
   int* s_lohi = s_tab + a.nblk * NW * kTabStride; // nblk x 4: chunk range, batch range
   int* s_cnt = s_lohi + 4 * a.nblk;  // nblk x B x kSweepRep: contributions per acc_out entry
   int* s_rtab = s_cnt + a.nblk * B * kSweepRep;  // kRoundTab x NW x kTabStride: next block's rounds
+  float* s_pr = reinterpret_cast<float*>(s_rtab + kRoundTab * NW * kTabStride);  // Pr_b
+  float* s_th = s_pr + B;                                                        // theta_b
 
   // stage the normalised centroids (same padded layouts as the row kernel)
   const int KpS = Kp + 16;
@@ -265,6 +267,10 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
   for (int q = tid; q < BK; q += kBlock) {
     s_O[q] = a.O[q];
     s_E[q] = a.E[q];
+  }
+  for (int b = tid; b < B; b += kBlock) {
+    s_pr[b] = a.Pr[b];
+    s_th[b] = a.theta[b];
   }
   const float* YTs = reinterpret_cast<const float*>(smem);
   const __bf16* YHs = reinterpret_cast<const __bf16*>(smem);
@@ -677,18 +683,24 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
         long long vo[U][kSweepRep];
         unsigned it = 0;
         while (true) {
-          bool ok = true;
+          // all loads first, then one count check without short-circuit
+          // branches (a branch per load would wait for each load in turn)
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const int q = min(q0 + u * kBlock, BK - 1);
+#pragma unroll
+            for (int r = 0; r < kSweepRep; ++r) vo[u][r] = load_agent(src + r * a.rep_stride + q);
+          }
+          int short_count = 0;
 #pragma unroll
           for (int u = 0; u < U; ++u) {
             const int q = min(q0 + u * kBlock, BK - 1);
             const int* c = cnt + (q / K) * kSweepRep;
 #pragma unroll
-            for (int r = 0; r < kSweepRep; ++r) {
-              vo[u][r] = load_agent(src + r * a.rep_stride + q);
-              ok = ok && ((vo[u][r] >> kCountShift) >= c[r]);
-            }
+            for (int r = 0; r < kSweepRep; ++r)
+              short_count |= (int)((vo[u][r] >> kCountShift) < (long long)c[r]);
           }
-          if (ok) break;
+          if (!short_count) break;
           __builtin_amdgcn_s_sleep(1);
           if (++it >= a.spin_limit) {
             __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -716,7 +728,7 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
     __syncthreads();
     for (int q = tid; q < BK; q += kBlock) {
       const int b = q / K, k = q - b * K;
-      const float pr = a.Pr[b];
+      const float pr = s_pr[b];
       float o = s_O[q] + (float)s_do[q];
       float e = s_E[q] + pr * (float)s_co[k];
       o = o - (float)s_di[q];
@@ -724,7 +736,7 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
       s_O[q] = o;
       s_E[q] = e;
       // the penalty is needed only for the batches this workgroup processes next
-      if (next && b >= pbl && b <= pbh) s_pen[q] = penalty_value(e, o, a.theta[b], a.stabilized);
+      if (next && b >= pbl && b <= pbh) s_pen[q] = penalty_value(e, o, s_th[b], a.stabilized);
     }
     __syncthreads();
   };
@@ -734,34 +746,31 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
     for (int k = tid; k < K; k += kBlock) s_ci[k] = 0.0;
   };
 
-  // block 0: remove its old rows, its penalty; first similarity tile; loads
-  // for block 1 in flight
+  // block 0: remove its old rows, its penalty; first similarity tile
   issue_in(0);
   issue_z(0);
   load_in(0);
   update(-1, true);
   take_prefetch(0);
   stage_rounds(0);
-  if (a.nblk > 1) {
-    issue_in(1);
-    issue_z(1);
-  }
 
   for (int j = 0; j < a.nblk; ++j) {
     const bool more = j + 1 < a.nblk;
+    // the next block's old-row deltas and first Z rows are loaded while this
+    // block's rows run, and used in the same iteration: nothing loaded is
+    // carried across the loop edge, where a register copy would wait for it
+    if (more) {
+      issue_in(j + 1);
+      issue_z(j + 1);
+    }
     rows_block(j);
     flush(j);
     // while the other workgroups' sums arrive: the next block's old-row
-    // deltas and similarity tile (both loaded a block ago), then the loads
-    // for the block after it
+    // deltas, similarity tile and round table
     if (more) {
       load_in(j + 1);
       take_prefetch(j + 1);
       stage_rounds(j + 1);
-      if (j + 2 < a.nblk) {
-        issue_in(j + 2);
-        issue_z(j + 2);
-      }
     } else {
       clear_in();
     }
