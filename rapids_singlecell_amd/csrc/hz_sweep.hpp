@@ -204,6 +204,19 @@ __host__ __device__ inline size_t sweep_lds(int Kp, int B, int K, int nblk) {
          (size_t)2 * B * 4 + 64;
 }
 
+// e^x as 2^(x log2 e) on the hardware exp2, with the rounding error of the
+// product (and of log2 e in FP32) carried into a first-order correction:
+// within a few ulp of expf for the arguments a row sees (|x| < 120).
+__device__ __forceinline__ float exp_rows(float x) {
+  constexpr float kLog2e = 1.44269502162933349609375f;   // log2(e) rounded to FP32
+  constexpr float kLog2eLo = 1.925963033500011e-08f;     // log2(e) - kLog2e
+  constexpr float kLn2 = 0.693147180559945309f;
+  const float t = x * kLog2e;
+  const float e = __builtin_fmaf(x, kLog2eLo, __builtin_fmaf(x, kLog2e, -t));
+  const float y = __builtin_amdgcn_exp2f(t);
+  return __builtin_fmaf(y, e * kLn2, y);
+}
+
 template <int PREC, int NT, int SM>
 __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -228,8 +241,8 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
   const size_t yb = align_up(row_mfma_lds<PREC, SM>(Kp), 16);
   float* s_O = reinterpret_cast<float*>(smem + yb);
   float* s_E = s_O + BK;
-  float* s_pen = s_E + BK;
-  double* s_do = reinterpret_cast<double*>(s_pen + BK);  // new-row deltas (update)
+  float* s_lpen = s_E + BK;  // log of the penalty
+  double* s_do = reinterpret_cast<double*>(s_lpen + BK);  // new-row deltas (update)
   float* s_dO = reinterpret_cast<float*>(s_do);          // per-block sums (rows), aliased
   double* s_di = s_do + BK;                              // old-row deltas of the next block
   double* s_co = s_di + BK;
@@ -402,14 +415,16 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
     sim_mfma(za, acc);
   };
 
-  // row epilogue of one 16-row tile (kernels_pen.cuh:27-82): lane holds rows 4*lg + r
+  // row epilogue of one 16-row tile (kernels_pen.cuh:27-82): lane holds rows 4*lg + r.
+  // exp(term (1 - sim)) * pen is taken as one exponential of term (1 - sim) +
+  // log pen; the normalisations use the hardware reciprocal (1 ulp).
   auto finish_tile = [&](int b, int my_cell, int len, int r0, const floatx4 (&acc)[NT],
                          float (&dO)[NT]) __attribute__((always_inline)) {
-    float pv[NT];
+    float lp[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
       const int k = 16 * t + lr;
-      pv[t] = (k < K) ? s_pen[b * K + k] : 0.f;
+      lp[t] = s_lpen[b * K + min(k, K - 1)];
     }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -419,11 +434,11 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
       float s = 0.f;
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
-        v[t] = (16 * t + lr < K) ? dexp(a.term * (1.f - acc[t][r])) * pv[t] : 0.f;
+        v[t] = (16 * t + lr < K) ? exp_rows(__builtin_fmaf(a.term, 1.f - acc[t][r], lp[t])) : 0.f;
         s += v[t];
       }
       s = sum16(s);
-      const float inv = 1.f / s;
+      const float inv = __builtin_amdgcn_rcpf(s);
       float s2 = 0.f;
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
@@ -431,7 +446,7 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
         s2 += v[t];
       }
       s2 = sum16(s2);
-      const float inv2 = 1.f / s2;
+      const float inv2 = __builtin_amdgcn_rcpf(s2);
       if (rr < len) {
         float* dst = a.R + (unsigned)rcell * (unsigned)K;
 #pragma unroll
@@ -736,7 +751,10 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
       s_O[q] = o;
       s_E[q] = e;
       // the penalty is needed only for the batches this workgroup processes next
-      if (next && b >= pbl && b <= pbh) s_pen[q] = penalty_value(e, o, s_th[b], a.stabilized);
+      if (next && b >= pbl && b <= pbh) {
+        const float denom = a.stabilized ? (o + e + 1.f) : (o + 1.f);
+        s_lpen[q] = s_th[b] * __builtin_logf((e + 1.f) / denom);
+      }
     }
     __syncthreads();
   };
