@@ -188,6 +188,7 @@ struct Layout {
   Ctrl* ctrl_static;  // never set done: used by the correction pass
   int* counters;      // nblk + 6
   int* swcnt;         // nblk * B * kSweepRep: block-sweep contribution counts
+  float* swtrash;     // 256 floats: stores of the sweep's padded lanes
   int Gs = 1;         // workgroups of the streaming centroid pass
   void* Ypart;
   void* Y;
@@ -313,6 +314,7 @@ static int make_layout(const hz_dims* dm, void* base, Layout& L) {
   L.ctrl_static = c.take<Ctrl>(1);
   L.counters = c.take<int>(L.pd.nblk + 6);
   L.swcnt = c.take<int>((long long)L.pd.nblk * L.B * kSweepRep);
+  L.swtrash = c.take<float>(256);
   L.Ypart = c.take<char>((size_t)L.G * KD * L.esz);
   L.Y = c.take<char>((size_t)KD * L.esz);
   L.YT = c.take<char>((size_t)L.d * L.Kp * L.esz);
@@ -936,7 +938,8 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
     // 16-byte loads of R and Z rows in the streaming centroid pass
     const bool aligned = (reinterpret_cast<uintptr_t>(Z) % 16 == 0) &&
                          (reinterpret_cast<uintptr_t>(R) % 16 == 0);
-    use_sw = use_rm && aligned && L.K <= 256 && L.pd.nblk <= kSweepMaxBlocks &&
+    // one 16-row MFMA tile per chunk
+    use_sw = use_rm && aligned && L.chunk == 16 && L.K <= 256 && L.pd.nblk <= kSweepMaxBlocks &&
              g_tune[HZ_TUNE_SWEEP] == 0 &&
              !g_sweep_broken[current_device()];
     sw.Z = Z;
@@ -976,6 +979,7 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
     sw.scale_obj = L.scale_obj;
     sw.inv_scale_obj = 1.0 / L.scale_obj;
     sw.cnt = L.swcnt;
+    sw.trash = L.swtrash;
     // counted block sums: value field below bit 53 holds at most N * 2^s
     sw.scale_pk = std::ldexp(1.0, std::min(40, 52 - bit_length((unsigned long long)L.n + 1)));
     sw.inv_scale_pk = 1.0 / sw.scale_pk;

@@ -152,6 +152,7 @@ struct SweepArgs {
   const __bf16* YH;  // Kp x dP (bf16 mode)
   const __bf16* YL;
   float* R;
+  float* trash;  // 16 * NT floats: destination of the stores of padded lanes
   const int* order;
   const int* seg_ch;
   const int* ch_start;
@@ -409,11 +410,6 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
       }
     }
   };
-  auto sim_tile = [&](int my_cell, int len, int r0, floatx4 (&acc)[NT]) __attribute__((always_inline)) {
-    float za[NA];
-    load_z(my_cell, len, r0, za);
-    sim_mfma(za, acc);
-  };
 
   // row epilogue of one 16-row tile (kernels_pen.cuh:27-82): lane holds rows 4*lg + r.
   // exp(term (1 - sim)) * pen is taken as one exponential of term (1 - sim) +
@@ -447,19 +443,21 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
       }
       s2 = sum16(s2);
       const float inv2 = __builtin_amdgcn_rcpf(s2);
-      if (rr < len) {
-        float* dst = a.R + (unsigned)rcell * (unsigned)K;
+      // every lane stores every value: rows past the chunk and padded
+      // columns go to a scratch row, so the store count is fixed and later
+      // waits on older loads need not wait for these stores
+      const bool row_ok = rr < len;
+      float* dst = row_ok ? a.R + (unsigned)rcell * (unsigned)K : a.trash;
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-          const int k = 16 * t + lr;
-          if (k < K) {
-            dst[k] = v[t];
-            kmf += v[t] * 2.f * (1.f - acc[t][r]);
-            const float xx = v[t] * inv2;
-            entf += xx * __logf(xx + 1e-12f);
-            dO[t] += v[t];
-          }
-        }
+      for (int t = 0; t < NT; ++t) {
+        const int k = 16 * t + lr;
+        const bool ok = row_ok && k < K;
+        (k < K ? dst : a.trash)[k] = v[t];
+        const float m = ok ? 1.f : 0.f;
+        const float xx = v[t] * inv2;
+        kmf += m * (v[t] * 2.f * (1.f - acc[t][r]));
+        entf += m * (xx * __logf(xx + 1e-12f));
+        dO[t] += m * v[t];
       }
     }
   };
@@ -535,7 +533,7 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
     wg_range(j, lo, hi);
     const int sb = j * B;
     for (int q = tid; q < BK; q += kBlock) s_dO[q] = 0.f;
-    float zn[NA], zc[NA];
+    float zc[NA];
     int cseg = -1, clen = 0, ccell = -1;
     auto round_chunk = [&](int r, int& seg, int& len, int& cell) __attribute__((always_inline)) {
       const int c = lo + r * NW + wv;
@@ -554,44 +552,41 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
     };
     int r = 0;
     for (int cb = lo; cb < hi; cb += NW, ++r) {
-      const int c = cb + wv;
-      int seg = -1;
-      // next round's chunk and Z rows (loads in flight during this round)
-      const bool next = cb + NW < hi;
-      int nseg2 = -1, nlen2 = 0, ncell2 = -1;
-      if (next) {
-        round_chunk(r + 1, nseg2, nlen2, ncell2);
-        if (cb + NW + wv < hi) load_z(ncell2, nlen2, 0, zn);
+      const bool mine = cb + wv < hi;  // wave-uniform
+      // this round's chunk: round 0 from the block prefetch (tile in n_sim),
+      // later rounds from the previous round (Z rows in zc)
+      int seg, len, cell;
+      floatx4 acc[NT];
+      if (r == 0) {
+        seg = n_seg;
+        len = n_len;
+        cell = n_cell;
+      } else {
+        seg = cseg;
+        len = clen;
+        cell = ccell;
+        if (mine) sim_mfma(zc, acc);
       }
-      if (c < hi) {
+      // next round's chunk; its Z rows load into zc (whose MFMAs are issued)
+      // while this round is normalised.  The loads are unconditional: past
+      // the last round they read row 0.
+      if (cb + NW < hi) {
+        round_chunk(r + 1, cseg, clen, ccell);
+      } else {
+        cseg = -1;
+        clen = 0;
+        ccell = -1;
+      }
+      load_z(ccell, clen, 0, zc);
+      if (mine) {
         float dO[NT];
 #pragma unroll
         for (int t = 0; t < NT; ++t) dO[t] = 0.f;
-        int len, cell;
-        floatx4 acc[NT];
-        if (r == 0) {
-          seg = n_seg;
-          len = n_len;
-          cell = n_cell;
-        } else {
-          seg = cseg;
-          len = clen;
-          cell = ccell;
-          sim_mfma(zc, acc);
-        }
         const int b = seg - sb;
-        for (int r0 = 0; r0 < len; r0 += 16) {
-          if (r0 == 0) {
-            if (r == 0)
-              finish_tile(b, cell, len, 0, n_sim, dO);
-            else
-              finish_tile(b, cell, len, 0, acc, dO);
-          } else {
-            floatx4 acc2[NT];
-            sim_tile(cell, len, r0, acc2);
-            finish_tile(b, cell, len, r0, acc2, dO);
-          }
-        }
+        if (r == 0)
+          finish_tile(b, cell, len, 0, n_sim, dO);
+        else
+          finish_tile(b, cell, len, 0, acc, dO);
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
           float x = dO[t];
@@ -600,13 +595,8 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
           const int k = 16 * t + lr;
           if (lg == 0 && k < K) s_part[wv][k] = x;
         }
-      }
-      if (next) {
-        cseg = nseg2;
-        clen = nlen2;
-        ccell = ncell2;
-#pragma unroll
-        for (int i = 0; i < NA; ++i) zc[i] = zn[i];
+      } else {
+        seg = -1;
       }
       if (lane == 0) s_seg[wv] = seg;
       __syncthreads();
