@@ -412,8 +412,12 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
   };
 
   // row epilogue of one 16-row tile (kernels_pen.cuh:27-82): lane holds rows 4*lg + r.
-  // exp(term (1 - sim)) * pen is taken as one exponential of term (1 - sim) +
-  // log pen; the normalisations use the hardware reciprocal (1 ulp).
+  // exp(term (1 - sim)) * pen is taken as one exponential of its log,
+  // a = term (1 - sim) + log pen; the normalisations use the hardware
+  // reciprocal (1 ulp).  The entropy term x log x of the twice-normalised
+  // value x = e^a / (s s2) uses log x = a - log s - log s2: two logarithms
+  // per row instead of one per value (x log(x + 1e-12) and x log x differ by
+  // less than 1e-12 per value).
   auto finish_tile = [&](int b, int my_cell, int len, int r0, const floatx4 (&acc)[NT],
                          float (&dO)[NT]) __attribute__((always_inline)) {
     float lp[NT];
@@ -426,11 +430,12 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
     for (int r = 0; r < 4; ++r) {
       const int rr = r0 + 4 * lg + r;
       const int rcell = __shfl(my_cell, rr & 63, kWave);
-      float v[NT];
+      float v[NT], arg[NT];
       float s = 0.f;
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
-        v[t] = (16 * t + lr < K) ? exp_rows(__builtin_fmaf(a.term, 1.f - acc[t][r], lp[t])) : 0.f;
+        arg[t] = __builtin_fmaf(a.term, 1.f - acc[t][r], lp[t]);
+        v[t] = (16 * t + lr < K) ? exp_rows(arg[t]) : 0.f;
         s += v[t];
       }
       s = sum16(s);
@@ -443,6 +448,7 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
       }
       s2 = sum16(s2);
       const float inv2 = __builtin_amdgcn_rcpf(s2);
+      const float lrow = -(__logf(s) + __logf(s2));
       // every lane stores every value: rows past the chunk and padded
       // columns go to a scratch row, so the store count is fixed and later
       // waits on older loads need not wait for these stores
@@ -456,7 +462,7 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
         const float m = ok ? 1.f : 0.f;
         const float xx = v[t] * inv2;
         kmf += m * (v[t] * 2.f * (1.f - acc[t][r]));
-        entf += m * (xx * __logf(xx + 1e-12f));
+        entf += m * (xx * (arg[t] + lrow));
         dO[t] += m * v[t];
       }
     }
