@@ -430,22 +430,27 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
     for (int r = 0; r < 4; ++r) {
       const int rr = r0 + 4 * lg + r;
       const int rcell = __shfl(my_cell, rr & 63, kWave);
-      float v[NT], arg[NT];
+      float v[NT], arg[NT], om[NT];
       float s = 0.f;
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
-        arg[t] = __builtin_fmaf(a.term, 1.f - acc[t][r], lp[t]);
+        om[t] = 1.f - acc[t][r];
+        arg[t] = __builtin_fmaf(a.term, om[t], lp[t]);
         v[t] = (16 * t + lr < K) ? exp_rows(arg[t]) : 0.f;
         s += v[t];
       }
       s = sum16(s);
       const float inv = __builtin_amdgcn_rcpf(s);
-      float s2 = 0.f;
+      // this lane's parts of the row sums (padded columns hold 0)
+      float s2 = 0.f, va = 0.f, kv = 0.f;
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
         v[t] *= inv;
         s2 += v[t];
+        va = __builtin_fmaf(v[t], arg[t], va);
+        kv = __builtin_fmaf(v[t], om[t], kv);
       }
+      const float s2_lane = s2;
       s2 = sum16(s2);
       const float inv2 = __builtin_amdgcn_rcpf(s2);
       const float lrow = -(__logf(s) + __logf(s2));
@@ -453,18 +458,18 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
       // columns go to a scratch row, so the store count is fixed and later
       // waits on older loads need not wait for these stores
       const bool row_ok = rr < len;
+      const float m = row_ok ? 1.f : 0.f;
       float* dst = row_ok ? a.R + (unsigned)rcell * (unsigned)K : a.trash;
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
         const int k = 16 * t + lr;
-        const bool ok = row_ok && k < K;
         (k < K ? dst : a.trash)[k] = v[t];
-        const float m = ok ? 1.f : 0.f;
-        const float xx = v[t] * inv2;
-        kmf += m * (v[t] * 2.f * (1.f - acc[t][r]));
-        entf += m * (xx * (arg[t] + lrow));
-        dO[t] += m * v[t];
+        dO[t] = __builtin_fmaf(m, v[t], dO[t]);
       }
+      // k-means term sum v 2 (1 - sim); entropy sum x log x with
+      // x = v / s2 and log x = arg - log s - log s2
+      kmf = __builtin_fmaf(2.f * m, kv, kmf);
+      entf = __builtin_fmaf(m * inv2, __builtin_fmaf(lrow, s2_lane, va), entf);
     }
   };
 
