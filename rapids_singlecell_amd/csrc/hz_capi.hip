@@ -174,8 +174,8 @@ struct Layout {
   PermDims spd; // static phi chunks
   // clustering
   PermBufs pb;
-  ChunkTab ct;
-  ChunkTab ct2;  // second block-order table: iteration it+1 is built while it runs
+  ChunkTab ct;   // block-order tables of nord iterations (tab_of)
+  int nord;
   int* nch_tmp;
   ChunkTab st;
   int* nch_tmp_s;
@@ -260,6 +260,13 @@ static int make_layout(const hz_dims* dm, void* base, Layout& L) {
     L.chunk_p = cp;
   }
   L.pd = perm_dims(L.n, L.bs, L.B, L.chunk);
+  {
+    // block orders built per set of launches: as many iterations as fit in
+    // about 1 GiB of workspace, between 2 and 16
+    const PermStride st = perm_stride(L.pd);
+    const long long per = 4 * (6 * st.n + 2 * st.bk + st.bk1 + 2 * st.th + 3 * st.seg + 3 * st.ch);
+    L.nord = (int)std::max<long long>(2, std::min<long long>(16, (1LL << 30) / std::max<long long>(per, 1)));
+  }
   L.sd = perm_dims(L.n, L.n, L.B, 64);  // static batch-grouped table: 64-row chunks
   L.spd = perm_dims(L.n, L.n, L.B, L.chunk_p);
   L.G = (int)std::min<long long>(512, L.pd.maxch);
@@ -286,9 +293,7 @@ static int make_layout(const hz_dims* dm, void* base, Layout& L) {
   const long long KD = (long long)L.K * L.d;
 
   Carve c(base);
-  carve_perm(c, L.pd, L.pb, L.ct);
-  carve_tab(c, L.pd, L.ct2);
-  L.nch_tmp = c.take<int>(nseg + 1);
+  carve_perm(c, L.pd, L.pb, L.ct, &L.nch_tmp, L.nord);
   {
     PermBufs dummy;
     (void)dummy;
@@ -740,6 +745,7 @@ struct SideStream {
   hipEvent_t start = nullptr, done = nullptr;
   hipEvent_t ready[2] = {nullptr, nullptr};
   hipEvent_t freed[2] = {nullptr, nullptr};
+  hipEvent_t tabs = nullptr;  // block-order tables of the next iterations are built
 };
 static thread_local SideStream t_side[64];
 
@@ -747,7 +753,8 @@ static int side_stream(SideStream** out) {
   SideStream& ss = t_side[current_device()];
   if (!ss.s) {
     HZ_HIP(hipStreamCreateWithFlags(&ss.s, hipStreamNonBlocking));
-    hipEvent_t* evs[] = {&ss.start, &ss.done, &ss.ready[0], &ss.ready[1], &ss.freed[0], &ss.freed[1]};
+    hipEvent_t* evs[] = {&ss.start, &ss.done, &ss.ready[0], &ss.ready[1],
+                         &ss.freed[0], &ss.freed[1], &ss.tabs};
     for (hipEvent_t* e : evs) HZ_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
   }
   *out = &ss;
@@ -820,6 +827,7 @@ static int init_impl(const hz_dims* dm, const T* Z, const T* Y0, const int* cats
   const int grid = (int)L.sd.maxch;
   return launch_row<T>(L, a, grid, s);
 }
+
 
 template <typename T>
 static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, const T* Pr,
@@ -995,9 +1003,6 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
     if (use_sw && launch_sweep(L, nullptr, &sweep_grid, s) != HZ_OK) use_sw = false;
   }
 
-  // block orders are built one iteration ahead on a side stream, into two
-  // alternating tables
-  ChunkTab tabs[2] = {L.ct, L.ct2};
   auto bind_tab = [&](const ChunkTab& t) {
     ca.order = t.order;
     ca.seg_ch = t.seg_ch;
@@ -1036,15 +1041,24 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
   ia.nseg = (int)nseg;
   ia.K = L.K;
   ia.scale_o = L.scale_o;
-  auto enqueue_order = [&](int it) -> int {
-    ProfScope ps(HZ_PROF_ORDER, s2);
-    return launch_block_order(L.pd, seed + (uint32_t)it, cats, L.pb, tabs[it & 1], L.nch_tmp,
-                              &L.ctrl->done, s2);
+  // Block orders of L.nord iterations at a time, in one set of launches on
+  // the caller's stream ahead of those iterations: built beside the sweep
+  // (on the side stream) they would slow it, since every block waits for its
+  // slowest workgroup.  The side stream then waits for the new tables.
+  auto build_orders = [&](int it0) -> int {
+    const int cnt = std::min(L.nord, max_iter - it0);
+    {
+      ProfScope ps(HZ_PROF_ORDER, s);
+      HZ_TRY(launch_block_order(L.pd, seed + (uint32_t)it0, cats, L.pb, L.ct, L.nch_tmp,
+                                nullptr, s, cnt));
+    }
+    HZ_HIP(hipEventRecord(ss->tabs, s));
+    HZ_HIP(hipStreamWaitEvent(s2, ss->tabs, 0));
+    return HZ_OK;
   };
   // the side stream starts after the work already queued on the caller's stream
   HZ_HIP(hipEventRecord(ss->start, s));
   HZ_HIP(hipStreamWaitEvent(s2, ss->start, 0));
-  HZ_TRY(enqueue_order(0));
 
   DoneChecks* dc = nullptr;
   HZ_TRY(checks_init(&dc));
@@ -1056,11 +1070,12 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
       HZ_TRY(checks_poll(*dc, 3, &done));
       if (done) break;
     }
-    // side stream: the old-row sums of this iteration's blocks (once R of the
-    // previous iteration is final), then the next iteration's block order
+    if (it % L.nord == 0) HZ_TRY(build_orders(it));
+    const ChunkTab t = tab_of(L.ct, L.pd, it % L.nord);
+    // side stream: the old-row sums of this iteration's blocks, once R of the
+    // previous iteration is final
     if (it >= 1) HZ_HIP(hipStreamWaitEvent(s2, ss->freed[(it - 1) & 1], 0));
     if (use_sw) {
-      const ChunkTab& t = tabs[it & 1];
       ia.order = t.order;
       ia.seg_ch = t.seg_ch;
       ia.ch_start = t.ch_start;
@@ -1075,8 +1090,7 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
       HZ_LAUNCHED("sweep_count_kernel");
     }
     HZ_HIP(hipEventRecord(ss->ready[it & 1], s2));
-    if (it + 1 < max_iter) HZ_TRY(enqueue_order(it + 1));  // table (it+1)&1 was freed above
-    bind_tab(tabs[it & 1]);
+    bind_tab(t);
     bool iter_done = false;
     if (use_sw) {
       // sweep path: centroids in natural cell order

@@ -61,6 +61,17 @@ struct PermDims {
   long long maxch;
 };
 
+// Element strides between the buffers of consecutive iterations when several
+// block orders are built in one set of launches (blockIdx.y = iteration).
+struct PermStride {
+  long long n;    // keys, tk, ti, perm, order
+  long long bk;   // bhist, bcur
+  long long bk1;  // bstart
+  long long th;   // thist, tbase
+  long long seg;  // seg_start, seg_ch, nch_tmp
+  long long ch;   // ch_start, ch_len, ch_seg
+};
+
 static inline PermDims perm_dims(int n, int bs, int nb, int chunk) {
   PermDims p;
   p.n = n;
@@ -85,32 +96,60 @@ static inline void carve_tab(Carve& c, const PermDims& p, ChunkTab& t) {
   t.ch_seg = c.take<int>(p.maxch);
 }
 
-static inline void carve_perm(Carve& c, const PermDims& p, PermBufs& b, ChunkTab& t) {
+static inline PermStride perm_stride(const PermDims& p) {
   const long long nbk = 1LL << p.nbk_bits;
   const long long nseg = (long long)p.nblk * p.nb;
-  b.keys = c.take<uint32_t>(p.n);
-  b.tk = c.take<uint32_t>(p.n);
-  b.ti = c.take<int>(p.n);
-  b.perm = c.take<int>(p.n);
-  b.bhist = c.take<int>(nbk);
-  b.bstart = c.take<int>(nbk + 1);
-  b.bcur = c.take<int>(nbk);
-  b.thist = c.take<int>(nseg * p.tpb + 1);
-  b.tbase = c.take<int>(nseg * p.tpb + 1);
-  t.order = c.take<int>(p.n);
-  t.seg_start = c.take<int>(nseg + 1);
-  t.seg_ch = c.take<int>(nseg + 1);
-  t.ch_start = c.take<int>(p.maxch);
-  t.ch_len = c.take<int>(p.maxch);
-  t.ch_seg = c.take<int>(p.maxch);
+  return PermStride{p.n, nbk, nbk + 1, nseg * p.tpb + 1, nseg + 1, p.maxch};
+}
+
+// Buffers of `I` iterations' block orders, each field strided by iteration
+// (PermStride); I = 1 for a single order.
+static inline void carve_perm(Carve& c, const PermDims& p, PermBufs& b, ChunkTab& t,
+                              int* nch_tmp_out[] = nullptr, int I = 1) {
+  const PermStride st = perm_stride(p);
+  b.keys = c.take<uint32_t>(st.n * I);
+  b.tk = c.take<uint32_t>(st.n * I);
+  b.ti = c.take<int>(st.n * I);
+  b.perm = c.take<int>(st.n * I);
+  b.bhist = c.take<int>(st.bk * I);
+  b.bstart = c.take<int>(st.bk1 * I);
+  b.bcur = c.take<int>(st.bk * I);
+  b.thist = c.take<int>(st.th * I);
+  b.tbase = c.take<int>(st.th * I);
+  t.order = c.take<int>(st.n * I);
+  t.seg_start = c.take<int>(st.seg * I);
+  t.seg_ch = c.take<int>(st.seg * I);
+  t.ch_start = c.take<int>(st.ch * I);
+  t.ch_len = c.take<int>(st.ch * I);
+  t.ch_seg = c.take<int>(st.ch * I);
+  if (nch_tmp_out) *nch_tmp_out = c.take<int>(st.seg * I);
+}
+
+// Table of iteration i of a batch.
+static inline ChunkTab tab_of(const ChunkTab& t, const PermDims& p, int i) {
+  const PermStride st = perm_stride(p);
+  ChunkTab r;
+  r.order = t.order + st.n * i;
+  r.seg_start = t.seg_start + st.seg * i;
+  r.seg_ch = t.seg_ch + st.seg * i;
+  r.ch_start = t.ch_start + st.ch * i;
+  r.ch_len = t.ch_len + st.ch * i;
+  r.ch_seg = t.ch_seg + st.ch * i;
+  return r;
 }
 
 // ------------------------------------------------------------------ kernels
 
+// Every kernel below handles iteration blockIdx.y of a batch: seed + y, and
+// buffers offset by y times their PermStride.
 __global__ void perm_keys_kernel(const int* __restrict__ done, int n, uint32_t seed,
                                  int shift, uint32_t* __restrict__ keys,
-                                 int* __restrict__ bhist) {
+                                 int* __restrict__ bhist, PermStride st) {
   if (done && *done) return;
+  const int y = blockIdx.y;
+  seed += (uint32_t)y;
+  keys += st.n * y;
+  bhist += st.bk * y;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     uint32_t k = pcg_key((uint32_t)i, seed);
     keys[i] = k;
@@ -170,16 +209,23 @@ __device__ void wg_exclusive_scan(const int* __restrict__ in, int* __restrict__ 
 
 __global__ __launch_bounds__(kScanThreads) void perm_bucket_scan_kernel(
     const int* __restrict__ done, const int* __restrict__ bhist, int* __restrict__ bstart,
-    int* __restrict__ bcur, int nbk) {
+    int* __restrict__ bcur, int nbk, PermStride st) {
   __shared__ int lds[kScanThreads];
   if (done && *done) return;
-  wg_exclusive_scan(bhist, bstart, bcur, nbk, lds);
+  const int y = blockIdx.y;
+  wg_exclusive_scan(bhist + st.bk * y, bstart + st.bk1 * y, bcur + st.bk * y, nbk, lds);
 }
 
 __global__ void perm_scatter_kernel(const int* __restrict__ done, int n, int shift,
                                     const uint32_t* __restrict__ keys, int* __restrict__ bcur,
-                                    uint32_t* __restrict__ tk, int* __restrict__ ti) {
+                                    uint32_t* __restrict__ tk, int* __restrict__ ti,
+                                    PermStride st) {
   if (done && *done) return;
+  const int y = blockIdx.y;
+  keys += st.n * y;
+  bcur += st.bk * y;
+  tk += st.n * y;
+  ti += st.n * y;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     uint32_t k = keys[i];
     int slot = atomicAdd(&bcur[k >> shift], 1);
@@ -200,9 +246,16 @@ constexpr int kRankLds = 2048;
 __global__ __launch_bounds__(256) void perm_rank_kernel(
     const int* __restrict__ done, int n, int shift, const uint32_t* __restrict__ tk,
     const int* __restrict__ ti, const int* __restrict__ bstart, int* __restrict__ perm,
-    const int* __restrict__ cats, int bs, int nb, int tpb, int* __restrict__ thist) {
+    const int* __restrict__ cats, int bs, int nb, int tpb, int* __restrict__ thist,
+    PermStride st) {
   __shared__ uint32_t sk[kRankLds];
   if (done && *done) return;
+  const int y = blockIdx.y;
+  tk += st.n * y;
+  ti += st.n * y;
+  bstart += st.bk1 * y;
+  perm += st.n * y;
+  if (thist) thist += st.th * y;
   const int s0 = blockIdx.x * kRankTile;
   const int s1 = min(n, s0 + kRankTile);
   // buckets are in slot order: the tile's keys lie in [L0, L1)
@@ -239,9 +292,14 @@ __global__ __launch_bounds__(256) void perm_rank_kernel(
 // deterministic.
 __global__ __launch_bounds__(64) void group_place_kernel(
     const int* __restrict__ done, const int* __restrict__ perm, const int* __restrict__ cats,
-    int n, int bs, int nb, int tpb, const int* __restrict__ tbase, int* __restrict__ order) {
+    int n, int bs, int nb, int tpb, const int* __restrict__ tbase, int* __restrict__ order,
+    PermStride st) {
   extern __shared__ int cnt[];
   if (done && *done) return;
+  const int y = blockIdx.y;
+  perm += st.n * y;
+  tbase += st.th * y;
+  order += st.n * y;
   const int j = blockIdx.x / tpb, t = blockIdx.x % tpb;
   const long long blk_end = std::min<long long>((long long)(j + 1) * bs, n);
   const long long p0 = (long long)j * bs + (long long)t * kGroupTile;
@@ -302,9 +360,15 @@ __global__ __launch_bounds__(kScanThreads) void chunk_table_kernel(
     const int* __restrict__ done, const int* __restrict__ thist, int* __restrict__ tbase,
     long long nseg, int tpb, int n, int chunk, int* __restrict__ seg_start,
     int* __restrict__ seg_ch, int* __restrict__ ch_start, int* __restrict__ ch_len,
-    int* __restrict__ ch_seg, int* __restrict__ nch_tmp) {
+    int* __restrict__ ch_seg, int* __restrict__ nch_tmp, PermStride st) {
   __shared__ int lds[kScanThreads];
   if (done && *done) return;
+  const int y = blockIdx.y;
+  if (thist) thist += st.th * y;
+  if (tbase) tbase += st.th * y;
+  seg_start += st.seg * y;
+  seg_ch += st.seg * y;
+  nch_tmp += st.seg * y;
   if (thist) {
     wg_exclusive_scan(thist, tbase, nullptr, nseg * tpb, lds);
     for (long long s = threadIdx.x; s < nseg; s += blockDim.x) seg_start[s] = tbase[s * tpb];
@@ -326,8 +390,14 @@ __global__ void chunk_fill_kernel(const int* __restrict__ done, long long nseg, 
                                   const int* __restrict__ seg_start,
                                   const int* __restrict__ seg_ch, int* __restrict__ ch_start,
                                   int* __restrict__ ch_len, int* __restrict__ ch_seg,
-                                  long long maxch) {
+                                  long long maxch, PermStride st) {
   if (done && *done) return;
+  const int y = blockIdx.y;
+  seg_start += st.seg * y;
+  seg_ch += st.seg * y;
+  ch_start += st.ch * y;
+  ch_len += st.ch * y;
+  ch_seg += st.ch * y;
   const long long total = seg_ch[nseg];
   for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < total && c < maxch;
        c += (long long)gridDim.x * blockDim.x) {
@@ -351,48 +421,53 @@ static inline int launch_grid_1d(long long n, int per_cu = 8) {
   return (int)std::max<long long>(1, std::min<long long>(g, (long long)device_cu_count() * per_cu));
 }
 
-// Key-sorted order for `seed` into b.perm (4 launches + 1 fill).  With
-// `cats`, the rank pass also fills the (block, batch, tile) histogram b.thist.
+// Key-sorted orders for seeds seed .. seed+I-1 into b.perm (iteration i at
+// b.perm + i * n).  With `cats`, the rank pass also fills the (block, batch,
+// tile) histograms b.thist.
 static int launch_key_order(const PermDims& p, uint32_t seed, PermBufs& b, const int* done,
-                            hipStream_t s, const int* cats = nullptr) {
+                            hipStream_t s, const int* cats = nullptr, int I = 1) {
   const int shift = 32 - p.nbk_bits;
   const int nbk = 1 << p.nbk_bits;
-  HZ_TRY(fill<int>(b.bhist, nbk, 0, s));
-  const int g = launch_grid_1d(p.n);
-  hipLaunchKernelGGL(perm_keys_kernel, dim3(g), dim3(256), 0, s, done, p.n, seed, shift, b.keys,
-                     b.bhist);
+  const PermStride st = perm_stride(p);
+  HZ_TRY(fill<int>(b.bhist, st.bk * I, 0, s));
+  const int g = launch_grid_1d(p.n, I > 1 ? 2 : 8);
+  hipLaunchKernelGGL(perm_keys_kernel, dim3(g, I), dim3(256), 0, s, done, p.n, seed, shift,
+                     b.keys, b.bhist, st);
   HZ_LAUNCHED("perm_keys");
-  hipLaunchKernelGGL(perm_bucket_scan_kernel, dim3(1), dim3(kScanThreads), 0, s, done, b.bhist,
-                     b.bstart, b.bcur, nbk);
+  hipLaunchKernelGGL(perm_bucket_scan_kernel, dim3(1, I), dim3(kScanThreads), 0, s, done,
+                     b.bhist, b.bstart, b.bcur, nbk, st);
   HZ_LAUNCHED("perm_bucket_scan");
-  hipLaunchKernelGGL(perm_scatter_kernel, dim3(g), dim3(256), 0, s, done, p.n, shift, b.keys,
-                     b.bcur, b.tk, b.ti);
+  hipLaunchKernelGGL(perm_scatter_kernel, dim3(g, I), dim3(256), 0, s, done, p.n, shift, b.keys,
+                     b.bcur, b.tk, b.ti, st);
   HZ_LAUNCHED("perm_scatter");
-  if (cats) HZ_TRY(fill<int>(b.thist, (long long)p.nblk * p.nb * p.tpb, 0, s));
-  hipLaunchKernelGGL(perm_rank_kernel, dim3((int)ceil_div(p.n, kRankTile)), dim3(256), 0, s,
-                     done, p.n, shift, b.tk, b.ti,
-                     b.bstart, b.perm, cats, p.bs, p.nb, p.tpb, cats ? b.thist : nullptr);
+  if (cats) HZ_TRY(fill<int>(b.thist, st.th * I, 0, s));
+  hipLaunchKernelGGL(perm_rank_kernel, dim3((int)ceil_div(p.n, kRankTile), I), dim3(256), 0, s,
+                     done, p.n, shift, b.tk, b.ti, b.bstart, b.perm, cats, p.bs, p.nb, p.tpb,
+                     cats ? b.thist : nullptr, st);
   HZ_LAUNCHED("perm_rank");
   return HZ_OK;
 }
 
-// Full block order + grouping + chunk table for one clustering iteration.
+// Block orders, grouping and chunk tables of I consecutive clustering
+// iterations (seeds seed .. seed+I-1) in one set of launches.
 static int launch_block_order(const PermDims& p, uint32_t seed, const int* cats, PermBufs& b,
-                              ChunkTab& t, int* nch_tmp, const int* done, hipStream_t s) {
-  HZ_TRY(launch_key_order(p, seed, b, done, s, cats));
+                              ChunkTab& t, int* nch_tmp, const int* done, hipStream_t s,
+                              int I = 1) {
+  HZ_TRY(launch_key_order(p, seed, b, done, s, cats, I));
   const long long nseg = (long long)p.nblk * p.nb;
   const size_t lds = (size_t)p.nb * sizeof(int);
   const int tiles = p.nblk * p.tpb;
-  hipLaunchKernelGGL(chunk_table_kernel, dim3(1), dim3(kScanThreads), 0, s, done, b.thist,
+  const PermStride st = perm_stride(p);
+  hipLaunchKernelGGL(chunk_table_kernel, dim3(1, I), dim3(kScanThreads), 0, s, done, b.thist,
                      b.tbase, nseg, p.tpb, p.n, p.chunk, t.seg_start, t.seg_ch, t.ch_start,
-                     t.ch_len, t.ch_seg, nch_tmp);
+                     t.ch_len, t.ch_seg, nch_tmp, st);
   HZ_LAUNCHED("chunk_table");
-  hipLaunchKernelGGL(chunk_fill_kernel, dim3(launch_grid_1d(p.maxch)), dim3(256), 0, s, done,
-                     nseg, p.chunk, t.seg_start, t.seg_ch, t.ch_start, t.ch_len, t.ch_seg,
-                     p.maxch);
+  hipLaunchKernelGGL(chunk_fill_kernel, dim3(launch_grid_1d(p.maxch, I > 1 ? 2 : 8), I),
+                     dim3(256), 0, s, done, nseg, p.chunk, t.seg_start, t.seg_ch, t.ch_start,
+                     t.ch_len, t.ch_seg, p.maxch, st);
   HZ_LAUNCHED("chunk_fill");
-  hipLaunchKernelGGL(group_place_kernel, dim3(tiles), dim3(64), lds, s, done, b.perm, cats, p.n,
-                     p.bs, p.nb, p.tpb, b.tbase, t.order);
+  hipLaunchKernelGGL(group_place_kernel, dim3(tiles, I), dim3(64), lds, s, done, b.perm, cats,
+                     p.n, p.bs, p.nb, p.tpb, b.tbase, t.order, st);
   HZ_LAUNCHED("group_place");
   return HZ_OK;
 }
@@ -414,14 +489,15 @@ static int launch_static_chunks(int n, int nb, int chunk, const int* cat_offsets
   hipLaunchKernelGGL(copy_int_kernel, dim3(1), dim3(256), 0, s, cat_offsets, t.seg_start,
                      (long long)nb + 1);
   HZ_LAUNCHED("copy_offsets");
+  const PermStride st{};  // a single table
   hipLaunchKernelGGL(chunk_table_kernel, dim3(1), dim3(kScanThreads), 0, s, nullptr, nullptr,
                      nullptr, (long long)nb, 1, n, chunk, t.seg_start, t.seg_ch, t.ch_start,
-                     t.ch_len, t.ch_seg, nch_tmp);
+                     t.ch_len, t.ch_seg, nch_tmp, st);
   HZ_LAUNCHED("chunk_table_static");
   const long long maxch = ceil_div(n, chunk) + nb;
   hipLaunchKernelGGL(chunk_fill_kernel, dim3(launch_grid_1d(maxch)), dim3(256), 0, s, nullptr,
                      (long long)nb, chunk, t.seg_start, t.seg_ch, t.ch_start, t.ch_len, t.ch_seg,
-                     maxch);
+                     maxch, st);
   HZ_LAUNCHED("chunk_fill_static");
   return HZ_OK;
 }

@@ -202,7 +202,7 @@ __host__ __device__ inline size_t sweep_lds(int Kp, int B, int K, int nblk) {
   return y + (size_t)3 * B * K * 4 + (size_t)2 * B * K * 8 + (size_t)2 * K * 8 +
          (size_t)nblk * (kBlock / kWave) * kTabStride * 4 + (size_t)nblk * 4 * 4 +
          (size_t)nblk * B * kSweepRep * 4 + (size_t)kRoundTab * (kBlock / kWave) * kTabStride * 4 +
-         (size_t)2 * B * 4 + 64;
+         (size_t)2 * B * 4 + 8 + (size_t)kBlock * 8 + 64;
 }
 
 // e^x as 2^(x log2 e) on the hardware exp2, with the rounding error of the
@@ -254,6 +254,8 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
   int* s_rtab = s_cnt + a.nblk * B * kSweepRep;  // kRoundTab x NW x kTabStride: next block's rounds
   float* s_pr = reinterpret_cast<float*>(s_rtab + kRoundTab * NW * kTabStride);  // Pr_b
   float* s_th = s_pr + B;                                                        // theta_b
+  double* s_colp = reinterpret_cast<double*>(  // kBlock column partials
+      smem + align_up((size_t)(reinterpret_cast<unsigned char*>(s_th + B) - smem), 8));
 
   // stage the normalised centroids (same padded layouts as the row kernel)
   const int KpS = Kp + 16;
@@ -490,14 +492,23 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
     n_cell = e[4 + lane];
     sim_mfma(zpre, n_sim);
   };
+  // Per-(batch, cluster) reads use a column-major thread mapping: thread tid
+  // handles cluster ck = tid % K and batches cg, cg + GB, ... (cg = tid / K),
+  // so that it can also sum its column without another pass over LDS.
+  const int GB = max(1, kBlock / K);
+  const int ck = tid % K, cg = tid / K;
+  const bool col_thread = tid < GB * K;
   // old-row deltas of a block, loaded one block ahead into registers
   constexpr int UI = 8;
   long long vin[UI];
-  const bool in_regs = BK <= UI * kBlock;
+  const bool in_regs = B <= UI * GB;
   auto issue_in = [&](int jn) __attribute__((always_inline)) {
     if (in_regs) {
 #pragma unroll
-      for (int u = 0; u < UI; ++u) vin[u] = a.acc_in[(long long)jn * BK + min(tid + u * kBlock, BK - 1)];
+      for (int u = 0; u < UI; ++u) {
+        const int b = min(cg + u * GB, B - 1);
+        vin[u] = a.acc_in[(long long)jn * BK + b * K + ck];
+      }
     }
   };
 
@@ -642,21 +653,32 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
 
   // old-row deltas of block jn (complete at launch) and their column sums
   auto load_in = [&](int jn) __attribute__((always_inline)) {
-    if (in_regs) {
+    double ci = 0.0;
+    if (col_thread) {
+      if (in_regs) {
 #pragma unroll
-      for (int u = 0; u < UI; ++u) {
-        const int q = tid + u * kBlock;
-        if (q < BK) s_di[q] = (double)vin[u] * inv_o;
+        for (int u = 0; u < UI; ++u) {
+          const int b = cg + u * GB;
+          if (b < B) {
+            const double x = (double)vin[u] * inv_o;
+            s_di[b * K + ck] = x;
+            ci += x;
+          }
+        }
+      } else {
+        for (int b = cg; b < B; b += GB) {
+          const double x = (double)a.acc_in[(long long)jn * BK + b * K + ck] * inv_o;
+          s_di[b * K + ck] = x;
+          ci += x;
+        }
       }
-    } else {
-      for (int q = tid; q < BK; q += kBlock)
-        s_di[q] = (double)a.acc_in[(long long)jn * BK + q] * inv_o;
+      s_colp[cg * K + ck] = ci;
     }
     __syncthreads();
     for (int k = tid; k < K; k += kBlock) {
-      double ci = 0.0;
-      for (int b = 0; b < B; ++b) ci += s_di[b * K + k];
-      s_ci[k] = ci;
+      double c = 0.0;
+      for (int g = 0; g < GB; ++g) c += s_colp[g * K + k];
+      s_ci[k] = c;
     }
   };
 
@@ -675,71 +697,64 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
     const int pbh = next ? s_lohi[4 * jn_pen + 3] : -1;
     if (jo < 0) {
       for (int q = tid; q < BK; q += kBlock) s_do[q] = 0.0;
+      for (int k = tid; k < K; k += kBlock) s_co[k] = 0.0;
     } else {
       const long long* src = a.acc_out + (long long)jo * BK;
       const int* cnt = s_cnt + jo * B * kSweepRep;
-      // wave 0 waits on one entry per (batch, replica) first, so that the
-      // full read below usually succeeds at once
-      if (wv == 0) {
-        for (int i = lane; i < B * kSweepRep; i += kWave) {
-          const int b = i / kSweepRep, r = i - b * kSweepRep;
-          const long long need = cnt[i];
+      // each thread reads its column's entries (all replicas), all loads
+      // first, then one count check without short-circuit branches (a branch
+      // per load would wait for each load in turn); retried until every
+      // contribution has arrived
+      constexpr int UB = 8;
+      double co = 0.0;
+      if (col_thread) {
+        for (int b0 = cg; b0 < B; b0 += UB * GB) {
+          long long vo[UB][kSweepRep];
           unsigned it = 0;
-          while ((load_agent(src + r * a.rep_stride + b * K) >> kCountShift) < need) {
+          while (true) {
+#pragma unroll
+            for (int u = 0; u < UB; ++u) {
+              const int b = min(b0 + u * GB, B - 1);
+#pragma unroll
+              for (int r = 0; r < kSweepRep; ++r)
+                vo[u][r] = load_agent(src + r * a.rep_stride + b * K + ck);
+            }
+            int short_count = 0;
+#pragma unroll
+            for (int u = 0; u < UB; ++u) {
+              const int b = min(b0 + u * GB, B - 1);
+#pragma unroll
+              for (int r = 0; r < kSweepRep; ++r)
+                short_count |= (int)((vo[u][r] >> kCountShift) < (long long)cnt[b * kSweepRep + r]);
+            }
+            if (!short_count) break;
             __builtin_amdgcn_s_sleep(1);
             if (++it >= a.spin_limit) {
               __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
               break;
             }
           }
+#pragma unroll
+          for (int u = 0; u < UB; ++u) {
+            const int b = b0 + u * GB;
+            if (b < B) {
+              long long so = 0;
+#pragma unroll
+              for (int r = 0; r < kSweepRep; ++r) so += vo[u][r] & kValueMask;
+              const double x = (double)so * a.inv_scale_pk;
+              s_do[b * K + ck] = x;
+              co += x;
+            }
+          }
         }
+        s_colp[cg * K + ck] = co;
       }
       __syncthreads();
-      for (int q0 = tid; q0 < BK; q0 += U * kBlock) {
-        long long vo[U][kSweepRep];
-        unsigned it = 0;
-        while (true) {
-          // all loads first, then one count check without short-circuit
-          // branches (a branch per load would wait for each load in turn)
-#pragma unroll
-          for (int u = 0; u < U; ++u) {
-            const int q = min(q0 + u * kBlock, BK - 1);
-#pragma unroll
-            for (int r = 0; r < kSweepRep; ++r) vo[u][r] = load_agent(src + r * a.rep_stride + q);
-          }
-          int short_count = 0;
-#pragma unroll
-          for (int u = 0; u < U; ++u) {
-            const int q = min(q0 + u * kBlock, BK - 1);
-            const int* c = cnt + (q / K) * kSweepRep;
-#pragma unroll
-            for (int r = 0; r < kSweepRep; ++r)
-              short_count |= (int)((vo[u][r] >> kCountShift) < (long long)c[r]);
-          }
-          if (!short_count) break;
-          __builtin_amdgcn_s_sleep(1);
-          if (++it >= a.spin_limit) {
-            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            break;
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const int q = q0 + u * kBlock;
-          if (q < BK) {
-            long long so = 0;
-#pragma unroll
-            for (int r = 0; r < kSweepRep; ++r) so += vo[u][r] & kValueMask;
-            s_do[q] = (double)so * a.inv_scale_pk;
-          }
-        }
+      for (int k = tid; k < K; k += kBlock) {
+        double c = 0.0;
+        for (int g = 0; g < GB; ++g) c += s_colp[g * K + k];
+        s_co[k] = c;
       }
-    }
-    __syncthreads();
-    for (int k = tid; k < K; k += kBlock) {
-      double co = 0.0;
-      for (int b = 0; b < B; ++b) co += s_do[b * K + k];
-      s_co[k] = co;
     }
     __syncthreads();
     for (int q = tid; q < BK; q += kBlock) {
