@@ -1221,7 +1221,34 @@ static int correction_impl(const hz_dims* dm, const T* X, const T* R, const T* O
   ca.scale_o = L.scale_o;
   {
     ProfScope ps(HZ_PROF_PHI, s);
-    HZ_TRY(launch_centroid<T>(L, ca, (int)L.spd.maxch, s));
+    bool done_mfma = false;
+    if constexpr (std::is_same<T, float>::value) {
+      if (L.mfma && L.cent_mfma) {
+        // FP32 storage: the same per-chunk partials on matrix cores (true
+        // FP32 products), whatever the clustering precision
+        CentMArgs cm{};
+        cm.Z = X;
+        cm.R = R;
+        cm.order = L.sp.order;
+        cm.seg_start = L.sp.seg_start;
+        cm.seg_ch = L.sp.seg_ch;
+        cm.ch_start = L.sp.ch_start;
+        cm.ch_len = L.sp.ch_len;
+        cm.ch_seg = L.sp.ch_seg;
+        cm.acc_in = L.rsum;
+        cm.Ypart = reinterpret_cast<float*>(L.phipart);
+        cm.ctrl = L.ctrl_static;
+        cm.nseg = L.B;
+        cm.cpw = 1;
+        cm.n = L.n;
+        cm.d = L.d;
+        cm.K = L.K;
+        cm.scale_o = L.scale_o;
+        HZ_TRY(launch_cent_mfma_p<kPrecF32>(L, cm, (int)L.spd.maxch, s));
+        done_mfma = true;
+      }
+    }
+    if (!done_mfma) HZ_TRY(launch_centroid<T>(L, ca, (int)L.spd.maxch, s));
   }
   {
     long long tot = (long long)BK * L.d;
