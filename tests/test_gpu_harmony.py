@@ -158,6 +158,38 @@ def test_clustering_fixed_iters(device, dtype, stabilized, n, block_proportion):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_clustering_past_one_order_batch(device, dtype):
+    """20 clustering iterations: the block orders of iterations 0-15 and 16-19
+    come from two separate batched builds.  FP64 must match the oracle (same
+    permutation every iteration); FP32 (block sweep) keeps the bookkeeping
+    invariants: rows of R sum to 1 and sum_b O = column sums of R."""
+    pb = Problem(n=2000, dtype=dtype)
+    lib, dm, ws, nb, bufs, R, E, O, obj = _gpu_init(pb, device, False)
+    Rr, Er, Or, _ = ho.init_clusters(pb.Z_norm, pb.Y0, sigma=0.1, Pr_b=pb.Pr_b, theta=pb.theta,
+                                     cats=pb.codes, n_batches=pb.B, stabilized=False)
+    R.copy_(_t(Rr, device))
+    E.copy_(_t(Er, device))
+    O.copy_(_t(Or, device))
+    last = torch.zeros(1, dtype=_tdt(dtype), device=device)
+    n_iter = ctypes.c_int(0)
+    _native.check(lib.hz_clustering_loop(ctypes.byref(dm), _p(bufs["Z"]), _p(R), _p(E), _p(O),
+                                         _p(bufs["Pr"]), _p(bufs["cats"]), _p(bufs["th"]), 0.1,
+                                         float("-inf"), 20, 99, 0, _p(last),
+                                         ctypes.byref(n_iter), _p(ws), nb, _s(device)), "loop")
+    assert n_iter.value == 20
+    Rg, Og = R.cpu().numpy().astype(np.float64), O.cpu().numpy().astype(np.float64)
+    if dtype == np.float64:
+        ho.clustering_loop(pb.Z_norm, Rr, Er, Or, pb.Pr_b, pb.codes, pb.theta, sigma=0.1,
+                           tol=float("-inf"), max_iter=20, block_size=int(pb.n * 0.05), seed=99,
+                           stabilized=False)
+        assert rel(Rg, Rr) < 1e-7
+        assert rel(Og, Or) < 1e-7
+    else:
+        np.testing.assert_allclose(Rg.sum(axis=1), 1.0, atol=1e-5)
+        np.testing.assert_allclose(Og.sum(axis=0), Rg.sum(axis=0), rtol=1e-4)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_clustering_convergence_rule(device, dtype):
     """Window-3 convergence (clustering.cu:380-387): same iteration count."""
     pb = Problem(n=3000, dtype=dtype, K=10)
