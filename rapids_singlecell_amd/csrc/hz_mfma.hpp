@@ -398,19 +398,33 @@ __global__ __launch_bounds__(kBlock) void centroid_mfma_kernel(CentMArgs a) {
   for (int j = 0; j < MT; ++j) acc[j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
   __shared__ int s_cell[TR];
-  // load rows [p, p + rows) of `order` into the LDS tiles (zero beyond rows / K / d+1)
+  // load rows [p, p + rows) of `order` into the LDS tiles (zero beyond rows / K / d+1).
+  // All of a thread's loads are issued before any is used (addresses clamped
+  // in range, masks applied at the LDS store), so a tile costs one round trip.
   auto load_tile = [&](int p, int rows) {
     if (tid < TR) s_cell[tid] = a.order[p + min(tid, rows - 1)];
     __syncthreads();
     // 32-bit element offsets (the host keeps N*K and N*d below 2^31) let the
     // loads use a scalar base plus a 32-bit vector offset
-#pragma unroll 8
+    float rv[MR], zv[MZ];
+#pragma unroll
+    for (int m = 0; m < MR; ++m) {
+      const int e = tid + m * kBlock;
+      const int r = e / LWR, k = e % LWR;
+      rv[m] = a.R[(unsigned)s_cell[r] * (unsigned)K + (unsigned)min(k, K - 1)];
+    }
+#pragma unroll
+    for (int m = 0; m < MZ; ++m) {
+      const int e = tid + m * kBlock;
+      const int r = e / LWZ, cc = e % LWZ;
+      zv[m] = a.Z[(unsigned)s_cell[r] * (unsigned)d + (unsigned)min(cc, d - 1)];
+    }
+#pragma unroll
     for (int m = 0; m < MR; ++m) {
       const int e = tid + m * kBlock;
       const int r = e / LWR, k = e % LWR;
       if (k < 16 * NT) {
-        const float x = a.R[(unsigned)s_cell[r] * (unsigned)K + (unsigned)min(k, K - 1)];
-        const float v = (r < rows && k < K) ? x : 0.f;
+        const float v = (r < rows && k < K) ? rv[m] : 0.f;
         if (PREC == kPrecF32) {
           Rs[r * SR + k] = v;
         } else {
@@ -420,13 +434,12 @@ __global__ __launch_bounds__(kBlock) void centroid_mfma_kernel(CentMArgs a) {
         }
       }
     }
-#pragma unroll 8
+#pragma unroll
     for (int m = 0; m < MZ; ++m) {
       const int e = tid + m * kBlock;
       const int r = e / LWZ, cc = e % LWZ;
       if (cc < 16 * NU) {
-        const float x = a.Z[(unsigned)s_cell[r] * (unsigned)d + (unsigned)min(cc, d - 1)];
-        const float v = (r < rows) ? (cc < d ? x : (cc == d ? 1.f : 0.f)) : 0.f;
+        const float v = (r < rows) ? (cc < d ? zv[m] : (cc == d ? 1.f : 0.f)) : 0.f;
         if (PREC == kPrecF32) {
           Zs[r * SZ + cc] = v;
         } else {
