@@ -296,10 +296,24 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(ApplyArgs<T> a) {
     __syncthreads();
     if (tid < kTileRows) idx_s[tid] = (tid < rows) ? a.order[p0 + r0 + tid] : -1;
     __syncthreads();
-    for (int e = tid; e < kTileRows * K; e += kBlock) {
-      const int r = e / K, k = e - r * K;
-      const int i = idx_s[r];
-      RT[k * kZtStride + r] = (i >= 0) ? a.R[(long long)i * K + k] : T(0);
+    // R tile: eight loads in flight per thread (clamped addresses, masked at
+    // the LDS store, so no load waits behind a branch)
+    for (int e0 = tid; e0 < kTileRows * K; e0 += 8 * kBlock) {
+      T v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = min(e0 + u * kBlock, kTileRows * K - 1);
+        const int r = e / K, k = e - r * K;
+        v[u] = a.R[(long long)max(idx_s[r], 0) * K + k];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = e0 + u * kBlock;
+        if (e < kTileRows * K) {
+          const int r = e / K, k = e - r * K;
+          RT[k * kZtStride + r] = (idx_s[r] >= 0) ? v[u] : T(0);
+        }
+      }
     }
     __syncthreads();
     T acc[4][DT];
@@ -324,11 +338,14 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(ApplyArgs<T> a) {
       const int row = (r < rows) ? idx_s[r] : -1;
       T z[DT];
       T ss = T(0);
+      T xv[DT];
+#pragma unroll
+      for (int q = 0; q < DT; ++q)
+        xv[q] = a.X[(long long)max(row, 0) * d + min(cg * DT + q, d - 1)];
 #pragma unroll
       for (int q = 0; q < DT; ++q) {
         const int cc = cg * DT + q;
-        T x = (row >= 0 && cc < d) ? a.X[(long long)row * d + cc] : T(0);
-        z[q] = (cc < d) ? (x - acc[i][q]) : T(0);
+        z[q] = (row >= 0 && cc < d) ? (xv[q] - acc[i][q]) : T(0);
         ss += z[q] * z[q];
       }
       ss = group_sum(ss, 16);
