@@ -1469,8 +1469,9 @@ static int kmeans_impl(const T* Z, long long n, int d, int k, uint32_t seed, int
       default: hipLaunchKernelGGL((kmeans_assign_kernel<T, 16>), dim3(L.G), dim3(kBlock), lds, s, a); break;
     }
     HZ_LAUNCHED("kmeans_assign");
-    hipLaunchKernelGGL(kmeans_update_kernel<T>, dim3((int)std::max<long long>(1, ceil_div((long long)k * d, 256))),
-                       dim3(256), 0, s, L.part, L.G, Y, k, d, std::ldexp(1.0, -32));
+    hipLaunchKernelGGL(kmeans_update_kernel<T>,
+                       dim3((int)std::max<long long>(1, ceil_div((long long)k * d, kBlock / kWave))),
+                       dim3(kBlock), 0, s, L.part, L.G, Y, k, d, std::ldexp(1.0, -32));
     HZ_LAUNCHED("kmeans_update");
   }
   return HZ_OK;

@@ -477,20 +477,28 @@ __global__ __launch_bounds__(kBlock) void kmeans_assign_kernel(KmArgs<T> a) {
 }
 
 // Reduce partials; new centroid = mean (clusters with no cells keep theirs).
-// Also refreshes YT / half for the next assignment.
 template <typename T>
-__global__ void kmeans_update_kernel(const long long* __restrict__ part, int G, T* __restrict__ Y,
-                                     int K, int d, double inv_scale) {
+__global__ __launch_bounds__(kBlock) void kmeans_update_kernel(const long long* __restrict__ part,
+                                                               int G, T* __restrict__ Y, int K,
+                                                               int d, double inv_scale) {
+  // one wave per output: lanes stride over the G partials, then an exact
+  // integer wave sum (the partials are fixed point)
   const int KD = K * d;
-  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < KD; o += gridDim.x * blockDim.x) {
-    const int k = o / d;
-    long long s = 0, cnt = 0;
-    for (int g = 0; g < G; ++g) {
-      s += part[(long long)g * (KD + K) + o];
-      cnt += part[(long long)g * (KD + K) + KD + k];
-    }
-    if (cnt > 0) Y[o] = T((double)s * inv_scale / (double)cnt);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int o = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+  if (o >= KD) return;
+  const int k = o / d;
+  long long s = 0, cnt = 0;
+  for (int g = lane; g < G; g += kWave) {
+    s += part[(long long)g * (KD + K) + o];
+    cnt += part[(long long)g * (KD + K) + KD + k];
   }
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    s += __shfl_xor(s, off, kWave);
+    cnt += __shfl_xor(cnt, off, kWave);
+  }
+  if (lane == 0 && cnt > 0) Y[o] = T((double)s * inv_scale / (double)cnt);
 }
 
 template <typename T>
