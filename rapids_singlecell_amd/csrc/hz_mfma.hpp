@@ -604,6 +604,18 @@ __global__ __launch_bounds__(kBlock) void centroid_stream_kernel(CentSArgs a) {
     const float* Rb = a.R + r0 * K;
     const float* Zb = a.Z + r0 * d;
     const int lenR = rows * K, lenZ = rows * d;
+    if (rows == TR) {
+      // full tile: 64 K and 64 d are multiples of 4, so every float4 at a
+      // clamped offset is in range; one batch of loads with no branches
+      // (the store skips offsets past the tile)
+#pragma unroll
+      for (int m = 0; m < MR4; ++m)
+        rr[m] = *reinterpret_cast<const float4*>(Rb + min(4 * (tid + m * kBlock), lenR - 4));
+#pragma unroll
+      for (int m = 0; m < MZ4; ++m)
+        zz[m] = *reinterpret_cast<const float4*>(Zb + min(4 * (tid + m * kBlock), lenZ - 4));
+      return;
+    }
 #pragma unroll
     for (int m = 0; m < MR4; ++m) {
       const int f = 4 * (tid + m * kBlock);
