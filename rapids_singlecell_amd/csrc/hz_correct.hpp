@@ -416,9 +416,23 @@ __global__ __launch_bounds__(kBlock) void kmeans_assign_kernel(KmArgs<T> a) {
     if (row0 >= a.n) break;
     const int rows = (int)min<long long>(kTileRows, a.n - row0);
     __syncthreads();
-    for (int e = tid; e < kTileRows * d; e += kBlock) {
-      const int r = e / d, cc = e - r * d;
-      ZTs[cc * kZtStride + r] = (r < rows) ? a.Z[(row0 + r) * d + cc] : T(0);
+    // eight loads in flight per thread (clamped rows, masked at the store)
+    for (int e0 = tid; e0 < kTileRows * d; e0 += 8 * kBlock) {
+      T v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = min(e0 + u * kBlock, kTileRows * d - 1);
+        const int r = e / d, cc = e - r * d;
+        v[u] = a.Z[(row0 + min(r, rows - 1)) * d + cc];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = e0 + u * kBlock;
+        if (e < kTileRows * d) {
+          const int r = e / d, cc = e - r * d;
+          ZTs[cc * kZtStride + r] = (r < rows) ? v[u] : T(0);
+        }
+      }
     }
     __syncthreads();
     T acc[4][CT];
