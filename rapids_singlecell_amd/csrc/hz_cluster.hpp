@@ -417,14 +417,24 @@ __global__ __launch_bounds__(kBlock) void yreduce_kernel(YRedArgs<T> a) {
   double* part = reinterpret_cast<double*>(smem);  // 4 waves x d
   for (int c = lane; c < d; c += kWave) {
     const T* src = a.Ypart + (long long)k * d + c;
-    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    int g = w;
-    for (; g + 28 < a.G; g += 32) {
+    // sixteen partials in flight per lane per step
+    double s[16];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) s[u] += (double)src[(long long)(g + 4 * u) * KD];
+    for (int u = 0; u < 16; ++u) s[u] = 0.0;
+    int g = w;
+    for (; g + 60 < a.G; g += 64) {
+      T v[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) v[u] = src[(long long)(g + 4 * u) * KD];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) s[u] += (double)v[u];
     }
     for (; g < a.G; g += 4) s[0] += (double)src[(long long)g * KD];
-    part[w * d + c] = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
+#pragma unroll
+    for (int h = 8; h > 0; h >>= 1)
+#pragma unroll
+      for (int u = 0; u < h; ++u) s[u] += s[u + h];
+    part[w * d + c] = s[0];
   }
   __syncthreads();
   // row sum of squares in T, as l2_row_normalize_kernel
