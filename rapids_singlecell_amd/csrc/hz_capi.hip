@@ -1164,7 +1164,7 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
   HZ_HIP(hipMemcpyAsync(&h, L.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s));
   HZ_HIP(hipStreamSynchronize(s));
   if (n_iter) *n_iter = h.n_obj;
-  HZ_RET_IF(h.pad0 != 0, HZ_ERR_LAUNCH, "block sweep: a grid barrier wait timed out");
+  HZ_RET_IF(h.pad0 != 0, HZ_ERR_LAUNCH, "block sweep: a wait for the other workgroups' block sums timed out");
   return HZ_OK;
 }
 
