@@ -21,6 +21,8 @@ namespace hz {
 constexpr int kScanThreads = 256;
 constexpr int kScanPer = 8;  // consecutive elements per thread and tile
 constexpr int kGroupTile = 1024;  // positions per tile in the grouping passes
+constexpr int kBoundCap = 1024;   // keys kept per boundary bucket (buckets hold ~32-64)
+constexpr int kMemberTile = 2048; // cells per tile in the membership grouping passes
 
 __device__ __forceinline__ uint32_t pcg_key(uint32_t i, uint32_t seed) {
   // PCG-RXS-M-XS 32-bit output permutation applied to (i ^ seed).
@@ -39,6 +41,9 @@ struct PermBufs {
   int* bcur;       // nbk
   int* thist;      // nseg * tpb + 1
   int* tbase;      // nseg * tpb + 1
+  uint32_t* blist; // nblk * kBoundCap: keys of the buckets holding block boundaries
+  int* bcnt;       // nblk
+  uint32_t* thr;   // nblk: key at each block boundary
 };
 
 struct ChunkTab {
@@ -70,6 +75,7 @@ struct PermStride {
   long long th;   // thist, tbase
   long long seg;  // seg_start, seg_ch, nch_tmp
   long long ch;   // ch_start, ch_len, ch_seg
+  long long nb;   // blist (nblk * kBoundCap), bcnt and thr (nblk)
 };
 
 static inline PermDims perm_dims(int n, int bs, int nb, int chunk) {
@@ -99,7 +105,9 @@ static inline void carve_tab(Carve& c, const PermDims& p, ChunkTab& t) {
 static inline PermStride perm_stride(const PermDims& p) {
   const long long nbk = 1LL << p.nbk_bits;
   const long long nseg = (long long)p.nblk * p.nb;
-  return PermStride{p.n, nbk, nbk + 1, nseg * p.tpb + 1, nseg + 1, p.maxch};
+  // thist / tbase hold (segment, tile) tables of both grouping schemes
+  const long long tiles = std::max<long long>(p.tpb, ceil_div(p.n, kMemberTile));
+  return PermStride{p.n, nbk, nbk + 1, nseg * tiles + 1, nseg + 1, p.maxch, p.nblk};
 }
 
 // Buffers of `I` iterations' block orders, each field strided by iteration
@@ -116,6 +124,9 @@ static inline void carve_perm(Carve& c, const PermDims& p, PermBufs& b, ChunkTab
   b.bcur = c.take<int>(st.bk * I);
   b.thist = c.take<int>(st.th * I);
   b.tbase = c.take<int>(st.th * I);
+  b.blist = c.take<uint32_t>(st.nb * kBoundCap * I);
+  b.bcnt = c.take<int>(st.nb * I);
+  b.thr = c.take<uint32_t>(st.nb * I);
   t.order = c.take<int>(st.n * I);
   t.seg_start = c.take<int>(st.seg * I);
   t.seg_ch = c.take<int>(st.seg * I);
@@ -414,6 +425,193 @@ __global__ void chunk_fill_kernel(const int* __restrict__ done, long long nseg, 
   }
 }
 
+// ------------------------------------------------------------------ block membership
+//
+// The clustering loop needs, per iteration, each block's cells grouped by
+// batch.  Its O/E sums are exact (fixed point) and every row's R depends only
+// on the penalty of its block, so the order of cells inside a block does not
+// change any result: only membership matters, block(i) = rank(key_i) / bs.
+// Membership follows from the keys at the block boundaries (rank j*bs): the
+// bucket histogram locates each in one bucket, whose few keys are ranked.  A
+// counting sort by (block, batch) in cell order then builds the grouped
+// order, with no scatter of keys and no full ranking.
+
+// largest b in [0, nbk) with bstart[b] <= P (the bucket holding rank P < n)
+__device__ __forceinline__ int bucket_of(const int* __restrict__ bstart, int nbk, int P) {
+  int lo = 0, hi = nbk - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (bstart[mid] <= P) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// bmark[b] = first boundary j (1-based) whose rank j*bs falls in bucket b
+// (bmark zeroed beforehand); bcnt zeroed.
+__global__ void bound_mark_kernel(const int* __restrict__ bstart, int* __restrict__ bmark,
+                                  int* __restrict__ bcnt, int nbk, int nblk, int bs,
+                                  PermStride st) {
+  const int y = blockIdx.y;
+  bstart += st.bk1 * y;
+  bmark += st.bk * y;
+  bcnt += st.nb * y;
+  for (int j = 1 + (int)threadIdx.x; j < nblk; j += blockDim.x) {
+    const int b = bucket_of(bstart, nbk, j * bs);
+    if (j == 1 || bucket_of(bstart, nbk, (j - 1) * bs) != b) bmark[b] = j;
+  }
+  for (int j = threadIdx.x; j < nblk; j += blockDim.x) bcnt[j] = 0;
+}
+
+// keys of the marked buckets, appended to the list of their first boundary
+__global__ void bound_collect_kernel(const uint32_t* __restrict__ keys,
+                                     const int* __restrict__ bmark, int* __restrict__ bcnt,
+                                     uint32_t* __restrict__ blist, int n, int shift,
+                                     PermStride st) {
+  const int y = blockIdx.y;
+  keys += st.n * y;
+  bmark += st.bk * y;
+  bcnt += st.nb * y;
+  blist += st.nb * kBoundCap * y;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t k = keys[i];
+    const int m = bmark[k >> shift];
+    if (m) {
+      const int slot = atomicAdd(&bcnt[m], 1);
+      if (slot < kBoundCap) blist[(long long)m * kBoundCap + slot] = k;
+    }
+  }
+}
+
+// thr[j] = key of rank j*bs: the (j*bs - bstart[b])-th smallest key of its
+// bucket b (keys are distinct).  One workgroup per boundary.
+__global__ __launch_bounds__(256) void bound_select_kernel(
+    const int* __restrict__ bstart, const int* __restrict__ bmark, const int* __restrict__ bcnt,
+    const uint32_t* __restrict__ blist, uint32_t* __restrict__ thr, int nbk, int bs,
+    PermStride st) {
+  __shared__ uint32_t sk[kBoundCap];
+  const int y = blockIdx.y;
+  bstart += st.bk1 * y;
+  bmark += st.bk * y;
+  bcnt += st.nb * y;
+  blist += st.nb * kBoundCap * y;
+  thr += st.nb * y;
+  const int j = blockIdx.x + 1;
+  const int P = j * bs;
+  const int b = bucket_of(bstart, nbk, P);
+  const int m = bmark[b];
+  const int cnt = min(bcnt[m], kBoundCap);
+  const int target = P - bstart[b];
+  for (int t = threadIdx.x; t < cnt; t += blockDim.x) sk[t] = blist[(long long)m * kBoundCap + t];
+  __syncthreads();
+  for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
+    const uint32_t k = sk[t];
+    int r = 0;
+    for (int u = 0; u < cnt; ++u) r += (sk[u] < k) ? 1 : 0;
+    if (r == target) thr[j] = k;
+  }
+}
+
+// block of a key: number of boundary keys thr[1..nblk-1] (increasing) <= key
+__device__ __forceinline__ int block_of(const uint32_t* __restrict__ sthr, int nblk, uint32_t k) {
+  int lo = 0, hi = nblk - 1;  // answer in [0, nblk-1]
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (sthr[mid] <= k) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// (segment, tile) histogram of the cells, tiles of T2 consecutive cells.
+// LDS: nseg counters, then nblk thresholds.
+__global__ __launch_bounds__(256) void member_count_kernel(
+    const uint32_t* __restrict__ keys, const uint32_t* __restrict__ thr,
+    const int* __restrict__ cats, int n, int nblk, int nb, int tiles, int T2,
+    int* __restrict__ thist, PermStride st) {
+  extern __shared__ int lds[];
+  const int y = blockIdx.y;
+  keys += st.n * y;
+  thr += st.nb * y;
+  thist += st.th * y;
+  const int nseg = nblk * nb;
+  int* hist = lds;
+  uint32_t* sthr = reinterpret_cast<uint32_t*>(lds + nseg);
+  for (int e = threadIdx.x; e < nseg; e += blockDim.x) hist[e] = 0;
+  for (int e = threadIdx.x; e < nblk; e += blockDim.x) sthr[e] = thr[e];
+  __syncthreads();
+  const int tile = blockIdx.x;
+  const long long c0 = (long long)tile * T2;
+  const long long c1 = min((long long)n, c0 + T2);
+  for (long long i = c0 + threadIdx.x; i < c1; i += blockDim.x)
+    atomicAdd(&hist[block_of(sthr, nblk, keys[i]) * nb + cats[i]], 1);
+  __syncthreads();
+  for (int e = threadIdx.x; e < nseg; e += blockDim.x) thist[(long long)e * tiles + tile] = hist[e];
+}
+
+// Stable placement in cell order: wave w takes quarter w of the tile; its
+// first slot in each segment is the tile's segment offset plus the counts of
+// the waves before it.  LDS: 4 x nseg slots, then nblk thresholds.
+__global__ __launch_bounds__(256) void member_place_kernel(
+    const uint32_t* __restrict__ keys, const uint32_t* __restrict__ thr,
+    const int* __restrict__ cats, int n, int nblk, int nb, int tiles, int T2,
+    const int* __restrict__ tbase, int* __restrict__ order, PermStride st) {
+  extern __shared__ int lds[];
+  const int y = blockIdx.y;
+  keys += st.n * y;
+  thr += st.nb * y;
+  tbase += st.th * y;
+  order += st.n * y;
+  const int nseg = nblk * nb;
+  int* slot = lds;  // [wave][segment]
+  uint32_t* sthr = reinterpret_cast<uint32_t*>(lds + 4 * nseg);
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  for (int e = tid; e < 4 * nseg; e += blockDim.x) slot[e] = 0;
+  for (int e = tid; e < nblk; e += blockDim.x) sthr[e] = thr[e];
+  __syncthreads();
+  const int tile = blockIdx.x;
+  const long long c0 = (long long)tile * T2;
+  const long long c1 = min((long long)n, c0 + T2);
+  const long long q = (c1 - c0 + 3) / 4;
+  const long long w0 = c0 + w * q, w1 = min(c1, w0 + q);
+  for (long long i = w0 + lane; i < w1; i += 64)
+    atomicAdd(&slot[w * nseg + block_of(sthr, nblk, keys[i]) * nb + cats[i]], 1);
+  __syncthreads();
+  for (int e = tid; e < nseg; e += blockDim.x) {
+    int run = tbase[(long long)e * tiles + tile];
+    for (int u = 0; u < 4; ++u) {
+      const int c = slot[u * nseg + e];
+      slot[u * nseg + e] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+  const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  int* myslot = slot + w * nseg;
+  for (long long base = w0; base < w1; base += 64) {
+    const long long i = base + lane;
+    const bool valid = i < w1;
+    const int sg = valid ? block_of(sthr, nblk, keys[i]) * nb + cats[i] : -1;
+    unsigned long long active = __ballot(valid);
+    while (active) {
+      const int leader = __builtin_ctzll(active);
+      const int s0 = __builtin_amdgcn_readlane(sg, leader);
+      const unsigned long long m = __ballot(sg == s0) & active;
+      const int start = myslot[s0];
+      if (sg == s0) order[start + __popcll(m & lt)] = (int)i;
+      if (lane == leader) myslot[s0] = start + __popcll(m);
+      active &= ~m;
+    }
+  }
+}
+
+// Grouped orders by membership for I iterations (the caller checks
+// member_fits), in tiles of kMemberTile cells.
+static inline bool member_fits(const PermDims& p) {
+  const long long nseg = (long long)p.nblk * p.nb;
+  return (4 * nseg + p.nblk) * (long long)sizeof(int) <= 65536;
+}
+
 // ------------------------------------------------------------------ host
 
 static inline int launch_grid_1d(long long n, int per_cu = 8) {
@@ -450,9 +648,58 @@ static int launch_key_order(const PermDims& p, uint32_t seed, PermBufs& b, const
 
 // Block orders, grouping and chunk tables of I consecutive clustering
 // iterations (seeds seed .. seed+I-1) in one set of launches.
+static int launch_member_order(const PermDims& p, uint32_t seed, const int* cats, PermBufs& b,
+                               ChunkTab& t, int* nch_tmp, hipStream_t s, int I) {
+  const int shift = 32 - p.nbk_bits;
+  const int nbk = 1 << p.nbk_bits;
+  const PermStride st = perm_stride(p);
+  const long long nseg = (long long)p.nblk * p.nb;
+  HZ_TRY(fill<int>(b.bhist, st.bk * I, 0, s));
+  const int g = launch_grid_1d(p.n, I > 1 ? 2 : 8);
+  hipLaunchKernelGGL(perm_keys_kernel, dim3(g, I), dim3(256), 0, s, nullptr, p.n, seed, shift,
+                     b.keys, b.bhist, st);
+  HZ_LAUNCHED("perm_keys");
+  hipLaunchKernelGGL(perm_bucket_scan_kernel, dim3(1, I), dim3(kScanThreads), 0, s, nullptr,
+                     b.bhist, b.bstart, b.bcur, nbk, st);
+  HZ_LAUNCHED("perm_bucket_scan");
+  if (p.nblk > 1) {
+    int* bmark = b.bcur;
+    HZ_TRY(fill<int>(bmark, st.bk * I, 0, s));
+    hipLaunchKernelGGL(bound_mark_kernel, dim3(1, I), dim3(256), 0, s, b.bstart, bmark, b.bcnt,
+                       nbk, p.nblk, p.bs, st);
+    HZ_LAUNCHED("bound_mark");
+    hipLaunchKernelGGL(bound_collect_kernel, dim3(g, I), dim3(256), 0, s, b.keys, bmark, b.bcnt,
+                       b.blist, p.n, shift, st);
+    HZ_LAUNCHED("bound_collect");
+    hipLaunchKernelGGL(bound_select_kernel, dim3(p.nblk - 1, I), dim3(256), 0, s, b.bstart,
+                       bmark, b.bcnt, b.blist, b.thr, nbk, p.bs, st);
+    HZ_LAUNCHED("bound_select");
+  }
+  const int T2 = kMemberTile;
+  const int tiles = (int)ceil_div(p.n, T2);
+  hipLaunchKernelGGL(member_count_kernel, dim3(tiles, I), dim3(256),
+                     (size_t)(nseg + p.nblk) * sizeof(int), s, b.keys, b.thr, cats, p.n, p.nblk,
+                     p.nb, tiles, T2, b.thist, st);
+  HZ_LAUNCHED("member_count");
+  hipLaunchKernelGGL(chunk_table_kernel, dim3(1, I), dim3(kScanThreads), 0, s, nullptr, b.thist,
+                     b.tbase, nseg, tiles, p.n, p.chunk, t.seg_start, t.seg_ch, t.ch_start,
+                     t.ch_len, t.ch_seg, nch_tmp, st);
+  HZ_LAUNCHED("chunk_table");
+  hipLaunchKernelGGL(chunk_fill_kernel, dim3(launch_grid_1d(p.maxch, I > 1 ? 2 : 8), I),
+                     dim3(256), 0, s, nullptr, nseg, p.chunk, t.seg_start, t.seg_ch, t.ch_start,
+                     t.ch_len, t.ch_seg, p.maxch, st);
+  HZ_LAUNCHED("chunk_fill");
+  hipLaunchKernelGGL(member_place_kernel, dim3(tiles, I), dim3(256),
+                     (size_t)(4 * nseg + p.nblk) * sizeof(int), s, b.keys, b.thr, cats, p.n,
+                     p.nblk, p.nb, tiles, T2, b.tbase, t.order, st);
+  HZ_LAUNCHED("member_place");
+  return HZ_OK;
+}
+
 static int launch_block_order(const PermDims& p, uint32_t seed, const int* cats, PermBufs& b,
                               ChunkTab& t, int* nch_tmp, const int* done, hipStream_t s,
                               int I = 1) {
+  if (!done && member_fits(p)) return launch_member_order(p, seed, cats, b, t, nch_tmp, s, I);
   HZ_TRY(launch_key_order(p, seed, b, done, s, cats, I));
   const long long nseg = (long long)p.nblk * p.nb;
   const size_t lds = (size_t)p.nb * sizeof(int);
