@@ -3,12 +3,14 @@
 // The reference shuffles cells by sorting pcg keys with CUB radix sort
 // (clustering.cu:292-305, kernels_clustering.cuh:106-117) and then walks the
 // sorted order in blocks of `block_size` rows.  Here the same key-sorted order
-// is produced by a bucket sort on the key's top bits followed by an exact
-// in-bucket rank (keys are distinct, so the order is unique and equals the
-// radix-sort order bit for bit).  Each block's rows are then grouped by batch
-// (stable in key order) and cut into chunks of at most `chunk` rows that never
-// straddle a (block, batch) segment, so every workgroup of the row kernel sees
-// one batch: its penalty row is uniform and its O-contribution is K values.
+// (hz_block_order) is produced by a bucket sort on the key's top bits followed
+// by an exact in-bucket rank (keys are distinct, so the order is unique and
+// equals the radix-sort order bit for bit).  The clustering loop needs only
+// block membership (see "block membership" below).  Either way each block's
+// rows are grouped by batch and cut into chunks of at most `chunk` rows that
+// never straddle a (block, batch) segment, so every workgroup of the row
+// kernel sees one batch: its penalty row is uniform and its O-contribution is
+// K values.
 #pragma once
 
 #include "hz_common.hpp"
