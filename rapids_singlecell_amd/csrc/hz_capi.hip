@@ -264,7 +264,9 @@ static int make_layout(const hz_dims* dm, void* base, Layout& L) {
     // block orders built per set of launches: as many iterations as fit in
     // about 1 GiB of workspace, between 2 and 16
     const PermStride st = perm_stride(L.pd);
-    const long long per = 4 * (6 * st.n + 2 * st.bk + st.bk1 + 2 * st.th + 3 * st.seg + 3 * st.ch);
+    const long long ranked = member_fits(L.pd) ? 0 : 3 * st.n;
+    const long long per = 4 * (3 * st.n + ranked + 2 * st.bk + st.bk1 + 2 * st.th + 3 * st.seg +
+                               3 * st.ch + st.nb * (kBoundCap + 2));
     L.nord = (int)std::max<long long>(2, std::min<long long>(16, (1LL << 30) / std::max<long long>(per, 1)));
   }
   L.sd = perm_dims(L.n, L.n, L.B, 64);  // static batch-grouped table: 64-row chunks
@@ -293,7 +295,7 @@ static int make_layout(const hz_dims* dm, void* base, Layout& L) {
   const long long KD = (long long)L.K * L.d;
 
   Carve c(base);
-  carve_perm(c, L.pd, L.pb, L.ct, &L.nch_tmp, L.nord);
+  carve_perm(c, L.pd, L.pb, L.ct, &L.nch_tmp, L.nord, !member_fits(L.pd));
   {
     PermBufs dummy;
     (void)dummy;

@@ -112,15 +112,23 @@ static inline PermStride perm_stride(const PermDims& p) {
   return PermStride{p.n, nbk, nbk + 1, nseg * tiles + 1, nseg + 1, p.maxch, p.nblk};
 }
 
+// Block membership needs its (segment) counters of four waves in 64 KiB of LDS.
+static inline bool member_fits(const PermDims& p) {
+  const long long nseg = (long long)p.nblk * p.nb;
+  return (4 * nseg + p.nblk) * (long long)sizeof(int) <= 65536;
+}
+
 // Buffers of `I` iterations' block orders, each field strided by iteration
 // (PermStride); I = 1 for a single order.
+// Without `ranked`, the buffers of the full ranking (tk, ti, perm) are left
+// out: block membership does not use them.
 static inline void carve_perm(Carve& c, const PermDims& p, PermBufs& b, ChunkTab& t,
-                              int* nch_tmp_out[] = nullptr, int I = 1) {
+                              int* nch_tmp_out[] = nullptr, int I = 1, bool ranked = true) {
   const PermStride st = perm_stride(p);
   b.keys = c.take<uint32_t>(st.n * I);
-  b.tk = c.take<uint32_t>(st.n * I);
-  b.ti = c.take<int>(st.n * I);
-  b.perm = c.take<int>(st.n * I);
+  b.tk = ranked ? c.take<uint32_t>(st.n * I) : nullptr;
+  b.ti = ranked ? c.take<int>(st.n * I) : nullptr;
+  b.perm = ranked ? c.take<int>(st.n * I) : nullptr;
   b.bhist = c.take<int>(st.bk * I);
   b.bstart = c.take<int>(st.bk1 * I);
   b.bcur = c.take<int>(st.bk * I);
@@ -609,10 +617,6 @@ __global__ __launch_bounds__(256) void member_place_kernel(
 
 // Grouped orders by membership for I iterations (the caller checks
 // member_fits), in tiles of kMemberTile cells.
-static inline bool member_fits(const PermDims& p) {
-  const long long nseg = (long long)p.nblk * p.nb;
-  return (4 * nseg + p.nblk) * (long long)sizeof(int) <= 65536;
-}
 
 // ------------------------------------------------------------------ host
 
