@@ -112,6 +112,22 @@ size_t hz_block_order_workspace_bytes(int n_cells);
 int hz_block_order(int n_cells, uint32_t seed, int32_t* order, void* ws, size_t ws_bytes,
                    void* stream);
 
+/* Block membership of n_iter consecutive clustering iterations (seeds seed ..
+ * seed+n_iter-1), built by the same kernels and in the same batched launches
+ * as hz_clustering_loop builds them: block_of_cell[y * n_cells + i] =
+ * rank(pcg(i ^ (seed + y))) / block_size, i.e. the block of cell i in the
+ * key-sorted order; order_out (optional, n_iter x n_cells) receives the
+ * grouped order (block-major, then batch).  Synchronises the stream.  Returns
+ * HZ_ERR_UNSUPPORTED if a block-boundary bucket overflowed its key list.
+ * Test binding of the membership the clustering loop uses in place of the
+ * reference's full sort + block slicing (_cuda/harmony/clustering/
+ * clustering.cu:292-319: idx_list_alt[j*block_size .. (j+1)*block_size)). */
+size_t hz_block_membership_workspace_bytes(int n_cells, int block_size, int n_batches,
+                                           int n_iter);
+int hz_block_membership(int n_cells, int block_size, int n_batches, const int32_t* cats,
+                        uint32_t seed, int n_iter, int32_t* block_of_cell, int32_t* order_out,
+                        void* ws, size_t ws_bytes, void* stream);
+
 /* Initial R, E, O and objective from centroids Y0 (n_clusters x n_pcs).
  * Replaces the body of _initialize_centroids after k-means
  * (_harmony/__init__.py:323-351): _calc_R (_fuses.py:6-8), L1 normalise,

@@ -113,7 +113,7 @@ def theta_array(theta, n_batches: int, dtype, *, tau: float = 0, n_b=None,
     precedence squares the exponential, not its argument.
     """
     dtype = np.dtype(dtype)
-    if isinstance(theta, (float, int)) and not isinstance(theta, bool):
+    if isinstance(theta, (float, int)):  # bool is an int: True -> 1.0
         th = np.ones(n_batches, dtype=dtype) * float(theta)
     elif isinstance(theta, (list, np.ndarray)):
         th = np.asarray(theta, dtype=dtype)

@@ -56,6 +56,8 @@ _SIGNATURES = {
     "hz_l2_normalize": (_i, [_i, _vp, _vp, _ll, _i, _vp]),
     "hz_block_order_workspace_bytes": (_sz, [_i]),
     "hz_block_order": (_i, [_i, _u32, _vp, _vp, _sz, _vp]),
+    "hz_block_membership_workspace_bytes": (_sz, [_i, _i, _i, _i]),
+    "hz_block_membership": (_i, [_i, _i, _i, _vp, _u32, _i, _vp, _vp, _vp, _sz, _vp]),
     "hz_init_clusters": (
         _i,
         [_P, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _i, _vp, _vp, _vp, _vp, _vp, _sz, _vp],

@@ -6,9 +6,7 @@
 #include <type_traits>
 #include <vector>
 
-#include "hz_correct.hpp"
-#include "hz_mfma.hpp"
-#include "hz_sweep.hpp"
+#include "hz_host.hpp"
 
 namespace hz {
 
@@ -28,14 +26,14 @@ void set_error(const char* fmt, ...) {
 static int g_lds[64];
 static int g_cus[64];
 
-static int current_device() {
+int current_device_index() {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) dev = 0;
   return std::max(0, std::min(dev, 63));
 }
 
 int device_lds_limit() {
-  int dev = current_device();
+  int dev = current_device_index();
   if (!g_lds[dev]) {
     int v = 0;
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
@@ -47,7 +45,7 @@ int device_lds_limit() {
 }
 
 int device_cu_count() {
-  int dev = current_device();
+  int dev = current_device_index();
   if (!g_cus[dev]) {
     int v = 0;
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
@@ -62,7 +60,7 @@ int device_cu_count() {
 // use at most (device limit minus its static LDS), the same value on every
 // call, so concurrent callers with different sizes never lower it under each
 // other; it is needed once static + dynamic LDS passes the 64 KiB default.
-static int prepare_lds(const void* kern, size_t lds) {
+int prepare_lds(const void* kern, size_t lds) {
   hipFuncAttributes fa{};
   HZ_HIP(hipFuncGetAttributes(&fa, kern));
   const size_t limit = (size_t)device_lds_limit();
@@ -160,49 +158,14 @@ static int fixed_shift(unsigned long long max_abs) {
 
 // Tuning knobs (0 = automatic), set through hz_set_tuning.
 static int g_tune[HZ_TUNE_KEYS];
+int tune_value(int key) { return g_tune[key]; }
 
-struct Layout {
-  int n, d, K, B, bs, esz;
-  int CT, Kp, DT, Dp;
-  int chunk, chunk_p, G, cpw;
-  // matrix-core path (FP32 storage)
-  int mfma, prec, NT, NU, dP, cent_mfma, Gm, cpwm;
-  void* YH;
-  void* YL;
-  PermDims pd;
-  PermDims sd;  // static row chunks (cells grouped by batch)
-  PermDims spd; // static phi chunks
-  // clustering
-  PermBufs pb;
-  ChunkTab ct;   // block-order tables of nord iterations (tab_of)
-  int nord;
-  int* nch_tmp;
-  ChunkTab st;
-  int* nch_tmp_s;
-  ChunkTab sp;
-  int* nch_tmp_p;
-  long long* acc_in;
-  long long* acc_out;  // kRep replicas
-  long long* obj_acc;  // kRep x 2
-  Ctrl* ctrl;
-  Ctrl* ctrl_static;  // never set done: used by the correction pass
-  int* counters;      // nblk + 6
-  int* swcnt;         // nblk * B * kSweepRep: block-sweep contribution counts
-  float* swtrash;     // 256 floats: stores of the sweep's padded lanes
-  int Gs = 1;         // workgroups of the streaming centroid pass
-  void* Ypart;
-  void* Y;
-  void* YT;
-  void* pen;
-  // correction
-  void* phipart;
-  double* Phi;
-  long long* rsum;
-  void* lam;
-  void* Wt;
-  size_t bytes;
-  double scale_o, scale_obj;
-};
+// cooperative launch of the block sweep refused on a device
+static bool g_sweep_broken[64];
+void mark_sweep_broken() { g_sweep_broken[current_device_index()] = true; }
+bool sweep_broken() { return g_sweep_broken[current_device_index()]; }
+
+// Layout is declared in hz_host.hpp
 
 static int make_layout(const hz_dims* dm, void* base, Layout& L) {
   HZ_RET_IF(!dm, HZ_ERR_ARG, "dims is NULL");
@@ -427,114 +390,6 @@ static int launch_centroid(const Layout& L, const CentArgs<T>& a, int grid, hipS
   }
 }
 
-// ---------------------------------------------------------------- matrix-core dispatch
-
-template <int PREC, int NT, int SM>
-static int launch_rm(RowMArgs a, int grid, size_t lds, hipStream_t s) {
-  auto kern = row_mfma_kernel<PREC, NT, SM>;
-  HZ_TRY(prepare_lds(reinterpret_cast<const void*>(kern), lds));
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), lds, s, a);
-  HZ_LAUNCHED("row_mfma_kernel");
-  return HZ_OK;
-}
-
-// k-steps of the row kernel: f32 4 columns per step, bf16 32 per step
-static int row_sm(const Layout& L) {
-  if (L.prec == HZ_PREC_SPLIT_BF16) return (int)ceil_div(L.d, 32);  // 1..4
-  const int s = (int)ceil_div(L.d, 4);
-  return s <= 4 ? 4 : s <= 8 ? 8 : s <= 13 ? 13 : s <= 16 ? 16 : s <= 25 ? 25 : 32;
-}
-
-template <int PREC, int SM>
-static int launch_rm_sm(const Layout& L, RowMArgs a, int grid, hipStream_t s) {
-  const size_t lds = std::max(row_mfma_lds<PREC, SM>(L.Kp), (size_t)block_update_min_lds<float>(L.B));
-  HZ_RET_IF(lds > (size_t)device_lds_limit(), HZ_ERR_UNSUPPORTED,
-            "row kernel needs %zu bytes of LDS (limit %d)", lds, device_lds_limit());
-  a.smem_bytes = (int)lds;
-  switch (L.NT) {
-    case 4: return launch_rm<PREC, 4, SM>(a, grid, lds, s);
-    case 7: return launch_rm<PREC, 7, SM>(a, grid, lds, s);
-    case 8: return launch_rm<PREC, 8, SM>(a, grid, lds, s);
-    default: return launch_rm<PREC, 16, SM>(a, grid, lds, s);
-  }
-}
-
-template <int PREC>
-static int launch_row_mfma_p(const Layout& L, RowMArgs a, int grid, hipStream_t s) {
-  if constexpr (PREC == kPrecBF16) {
-    switch (row_sm(L)) {
-      case 1: return launch_rm_sm<PREC, 1>(L, a, grid, s);
-      case 2: return launch_rm_sm<PREC, 2>(L, a, grid, s);
-      case 3: return launch_rm_sm<PREC, 3>(L, a, grid, s);
-      default: return launch_rm_sm<PREC, 4>(L, a, grid, s);
-    }
-  } else {
-    switch (row_sm(L)) {
-      case 4: return launch_rm_sm<PREC, 4>(L, a, grid, s);
-      case 8: return launch_rm_sm<PREC, 8>(L, a, grid, s);
-      case 13: return launch_rm_sm<PREC, 13>(L, a, grid, s);
-      case 16: return launch_rm_sm<PREC, 16>(L, a, grid, s);
-      case 25: return launch_rm_sm<PREC, 25>(L, a, grid, s);
-      default: return launch_rm_sm<PREC, 32>(L, a, grid, s);
-    }
-  }
-}
-
-static int launch_row_mfma(const Layout& L, const RowMArgs& a, int grid, hipStream_t s) {
-  if (L.prec == HZ_PREC_SPLIT_BF16) return launch_row_mfma_p<kPrecBF16>(L, a, grid, s);
-  return launch_row_mfma_p<kPrecF32>(L, a, grid, s);
-}
-
-// ---------------------------------------------------------------- block sweep
-
-static bool g_sweep_broken[64];  // cooperative launch refused on this device
-
-// Returns HZ_OK after a launch, HZ_ERR_UNSUPPORTED (nothing launched) when the
-// grid cannot be made co-resident; other codes are errors.
-// a == nullptr: only choose the grid (*G, co-resident workgroups); else launch
-// with *G workgroups.
-template <int PREC, int NT, int SM>
-static int launch_sw(const Layout& L, const SweepArgs* a, int* Gp, hipStream_t s) {
-  auto kern = sweep_kernel<PREC, NT, SM>;
-  const size_t lds = sweep_lds<PREC, SM>(L.Kp, L.B, L.K, L.pd.nblk);
-  if (prepare_lds(reinterpret_cast<const void*>(kern), lds) != HZ_OK) return HZ_ERR_UNSUPPORTED;
-  int per_cu = 0;
-  HZ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern),
-                                                      kBlock, lds));
-  if (per_cu < 1) return HZ_ERR_UNSUPPORTED;
-  if (!a) {
-    const long long cap = (long long)per_cu * device_cu_count();
-    long long G = g_tune[HZ_TUNE_SWEEP_WG] > 0 ? g_tune[HZ_TUNE_SWEEP_WG] : device_cu_count();
-    *Gp = (int)std::max<long long>(1, std::min(G, cap));
-    return HZ_OK;
-  }
-  if (g_tune[HZ_TUNE_SWEEP_PLAIN]) {
-    // same grid, within the co-resident capacity computed above
-    hipLaunchKernelGGL(kern, dim3((unsigned)*Gp), dim3(kBlock), lds, s, *a);
-    HZ_LAUNCHED("sweep_kernel");
-    return HZ_OK;
-  }
-  void* args[] = {const_cast<SweepArgs*>(a)};
-  hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kern), dim3((unsigned)*Gp),
-                                            dim3(kBlock), args, (unsigned)lds, s);
-  if (e != hipSuccess) {
-    (void)hipGetLastError();
-    g_sweep_broken[current_device()] = true;
-    return HZ_ERR_UNSUPPORTED;
-  }
-  return HZ_OK;
-}
-
-template <int PREC, int SM>
-static int launch_sw_sm(const Layout& L, const SweepArgs* a, int* G, hipStream_t s) {
-  switch (L.NT) {
-    case 4: return launch_sw<PREC, 4, SM>(L, a, G, s);
-    case 7: return launch_sw<PREC, 7, SM>(L, a, G, s);
-    case 8: return launch_sw<PREC, 8, SM>(L, a, G, s);
-    default: return launch_sw<PREC, 16, SM>(L, a, G, s);
-  }
-}
-
 static int launch_insum(const Layout& L, const InSumArgs& a, hipStream_t s) {
   const int grid = (int)std::max<long long>(1, ceil_div(L.pd.maxch, kBlock / kWave));
   switch ((int)ceil_div(L.K, kWave)) {
@@ -546,108 +401,6 @@ static int launch_insum(const Layout& L, const InSumArgs& a, hipStream_t s) {
   }
   HZ_LAUNCHED("insum_kernel");
   return HZ_OK;
-}
-
-static int launch_sweep(const Layout& L, const SweepArgs* a, int* G, hipStream_t s) {
-  if (L.prec == HZ_PREC_SPLIT_BF16) {
-    switch (row_sm(L)) {
-      case 1: return launch_sw_sm<kPrecBF16, 1>(L, a, G, s);
-      case 2: return launch_sw_sm<kPrecBF16, 2>(L, a, G, s);
-      case 3: return launch_sw_sm<kPrecBF16, 3>(L, a, G, s);
-      default: return launch_sw_sm<kPrecBF16, 4>(L, a, G, s);
-    }
-  }
-  switch (row_sm(L)) {
-    case 4: return launch_sw_sm<kPrecF32, 4>(L, a, G, s);
-    case 8: return launch_sw_sm<kPrecF32, 8>(L, a, G, s);
-    case 13: return launch_sw_sm<kPrecF32, 13>(L, a, G, s);
-    case 16: return launch_sw_sm<kPrecF32, 16>(L, a, G, s);
-    case 25: return launch_sw_sm<kPrecF32, 25>(L, a, G, s);
-    default: return launch_sw_sm<kPrecF32, 32>(L, a, G, s);
-  }
-}
-
-template <int PREC, int NT, int NU>
-static int launch_cs(const CentSArgs& a, int grid, hipStream_t s) {
-  {
-    auto kern = centroid_stream_kernel<PREC, NT, NU>;
-    const size_t lds = stream_tile_lds<NT, NU>();
-    HZ_RET_IF(lds > (size_t)device_lds_limit(), HZ_ERR_UNSUPPORTED,
-              "centroid pass needs %zu bytes of LDS", lds);
-    HZ_TRY(prepare_lds(reinterpret_cast<const void*>(kern), lds));
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), lds, s, a);
-    HZ_LAUNCHED("centroid_stream_kernel");
-    return HZ_OK;
-  }
-}
-
-template <int PREC, int NT>
-static int launch_cs_nt(const Layout& L, const CentSArgs& a, int grid, hipStream_t s) {
-  switch (L.NU) {
-    case 2: return launch_cs<PREC, NT, 2>(a, grid, s);
-    case 3: return launch_cs<PREC, NT, 3>(a, grid, s);
-    case 4: return launch_cs<PREC, NT, 4>(a, grid, s);
-    case 7: return launch_cs<PREC, NT, 7>(a, grid, s);
-    default: return launch_cs<PREC, NT, 8>(a, grid, s);
-  }
-}
-
-template <int PREC>
-static int launch_cs_p(const Layout& L, const CentSArgs& a, int grid, hipStream_t s) {
-  switch (L.NT) {
-    case 4: return launch_cs_nt<PREC, 4>(L, a, grid, s);
-    case 7: return launch_cs_nt<PREC, 7>(L, a, grid, s);
-    case 8: return launch_cs_nt<PREC, 8>(L, a, grid, s);
-    default: return launch_cs_nt<PREC, 16>(L, a, grid, s);
-  }
-}
-
-static int launch_cent_stream(const Layout& L, const CentSArgs& a, int grid, hipStream_t s) {
-  if (L.prec == HZ_PREC_SPLIT_BF16) return launch_cs_p<kPrecBF16>(L, a, grid, s);
-  return launch_cs_p<kPrecF32>(L, a, grid, s);
-}
-
-template <int PREC, int NT, int NU>
-static int launch_cm(const CentMArgs& a, int grid, hipStream_t s) {
-  if constexpr (NT * NU <= 32) {
-    auto kern = centroid_mfma_kernel<PREC, NT, NU>;
-    const size_t lds = centroid_tile_lds<PREC, NT, NU>();
-    HZ_RET_IF(lds > (size_t)device_lds_limit(), HZ_ERR_UNSUPPORTED,
-              "centroid pass needs %zu bytes of LDS", lds);
-    HZ_TRY(prepare_lds(reinterpret_cast<const void*>(kern), lds));
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), lds, s, a);
-    HZ_LAUNCHED("centroid_mfma_kernel");
-    return HZ_OK;
-  } else {
-    set_error("centroid tile shape %d x %d not instantiated", NT, NU);
-    return HZ_ERR_UNSUPPORTED;
-  }
-}
-
-template <int PREC, int NT>
-static int launch_cm_nt(const Layout& L, const CentMArgs& a, int grid, hipStream_t s) {
-  switch (L.NU) {
-    case 2: return launch_cm<PREC, NT, 2>(a, grid, s);
-    case 3: return launch_cm<PREC, NT, 3>(a, grid, s);
-    case 4: return launch_cm<PREC, NT, 4>(a, grid, s);
-    case 7: return launch_cm<PREC, NT, 7>(a, grid, s);
-    default: return launch_cm<PREC, NT, 8>(a, grid, s);
-  }
-}
-
-template <int PREC>
-static int launch_cent_mfma_p(const Layout& L, const CentMArgs& a, int grid, hipStream_t s) {
-  switch (L.NT) {
-    case 4: return launch_cm_nt<PREC, 4>(L, a, grid, s);
-    case 7: return launch_cm_nt<PREC, 7>(L, a, grid, s);
-    case 8: return launch_cm_nt<PREC, 8>(L, a, grid, s);
-    default: return launch_cm_nt<PREC, 16>(L, a, grid, s);
-  }
-}
-
-static int launch_cent_mfma(const Layout& L, const CentMArgs& a, int grid, hipStream_t s) {
-  if (L.prec == HZ_PREC_SPLIT_BF16) return launch_cent_mfma_p<kPrecBF16>(L, a, grid, s);
-  return launch_cent_mfma_p<kPrecF32>(L, a, grid, s);
 }
 
 static int launch_split(const Layout& L, const Ctrl* ctrl, hipStream_t s) {
@@ -696,7 +449,7 @@ struct DoneChecks {
 static thread_local DoneChecks t_checks[64];
 
 static int checks_init(DoneChecks** out) {
-  DoneChecks& c = t_checks[current_device()];
+  DoneChecks& c = t_checks[current_device_index()];
   if (!c.host) {
     HZ_HIP(hipHostMalloc(reinterpret_cast<void**>(&c.host), DoneChecks::kSlots * sizeof(int)));
     for (auto& e : c.ev) HZ_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -752,7 +505,7 @@ struct SideStream {
 static thread_local SideStream t_side[64];
 
 static int side_stream(SideStream** out) {
-  SideStream& ss = t_side[current_device()];
+  SideStream& ss = t_side[current_device_index()];
   if (!ss.s) {
     HZ_HIP(hipStreamCreateWithFlags(&ss.s, hipStreamNonBlocking));
     hipEvent_t* evs[] = {&ss.start, &ss.done, &ss.ready[0], &ss.ready[1],
@@ -951,7 +704,7 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
     // one 16-row MFMA tile per chunk
     use_sw = use_rm && aligned && L.chunk == 16 && L.K <= 256 && L.pd.nblk <= kSweepMaxBlocks &&
              g_tune[HZ_TUNE_SWEEP] == 0 &&
-             !g_sweep_broken[current_device()];
+             !sweep_broken();
     sw.Z = Z;
     sw.YT = reinterpret_cast<const float*>(L.YT);
     sw.YH = reinterpret_cast<const __bf16*>(L.YH);
@@ -1052,7 +805,7 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
     {
       ProfScope ps(HZ_PROF_ORDER, s);
       HZ_TRY(launch_block_order(L.pd, seed + (uint32_t)it0, cats, L.pb, L.ct, L.nch_tmp,
-                                nullptr, s, cnt));
+                                nullptr, &L.ctrl->order_err, s, cnt));
     }
     HZ_HIP(hipEventRecord(ss->tabs, s));
     HZ_HIP(hipStreamWaitEvent(s2, ss->tabs, 0));
@@ -1166,6 +919,8 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
   HZ_HIP(hipMemcpyAsync(&h, L.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s));
   HZ_HIP(hipStreamSynchronize(s));
   if (n_iter) *n_iter = h.n_obj;
+  HZ_RET_IF(h.order_err != 0, HZ_ERR_UNSUPPORTED,
+            "block order: a block-boundary bucket held more than %d keys", kBoundCap);
   HZ_RET_IF(h.pad0 != 0, HZ_ERR_LAUNCH, "block sweep: a wait for the other workgroups' block sums timed out");
   return HZ_OK;
 }
@@ -1246,7 +1001,7 @@ static int correction_impl(const hz_dims* dm, const T* X, const T* R, const T* O
         cm.d = L.d;
         cm.K = L.K;
         cm.scale_o = L.scale_o;
-        HZ_TRY(launch_cent_mfma_p<kPrecF32>(L, cm, (int)L.spd.maxch, s));
+        HZ_TRY(launch_cent_mfma_f32(L, cm, (int)L.spd.maxch, s));
         done_mfma = true;
       }
     }
@@ -1569,6 +1324,60 @@ int hz_block_order(int n_cells, uint32_t seed, int32_t* order, void* ws, size_t 
   carve_perm(c, p, b, t);
   b.perm = order;
   return launch_key_order(p, seed, b, nullptr, s);
+}
+
+// Block membership exactly as the clustering loop builds it (same dims, same
+// kernels, n_iter iterations per set of launches).
+static PermDims member_dims(int n, int bs, int B) { return perm_dims(n, std::min(bs, n), B, 16); }
+
+size_t hz_block_membership_workspace_bytes(int n_cells, int block_size, int n_batches,
+                                           int n_iter) {
+  if (n_cells < 1 || block_size < 1 || n_batches < 1 || n_iter < 1) return 0;
+  const PermDims p = member_dims(n_cells, block_size, n_batches);
+  Carve c(nullptr);
+  PermBufs b;
+  ChunkTab t;
+  int* tmp = nullptr;
+  carve_perm(c, p, b, t, &tmp, n_iter, !member_fits(p));
+  (void)c.take<int>(64);  // error flag
+  return align_up(c.off, 256);
+}
+
+int hz_block_membership(int n_cells, int block_size, int n_batches, const int32_t* cats,
+                        uint32_t seed, int n_iter, int32_t* block_of_cell, int32_t* order_out,
+                        void* ws, size_t ws_bytes, void* stream) {
+  HZ_RET_IF(n_cells < 1 || block_size < 1 || n_batches < 1 || n_iter < 1, HZ_ERR_ARG,
+            "bad shape (n_cells=%d block_size=%d n_batches=%d n_iter=%d)", n_cells, block_size,
+            n_batches, n_iter);
+  HZ_RET_IF(!cats || !block_of_cell, HZ_ERR_ARG, "cats and block_of_cell are required");
+  const size_t need = hz_block_membership_workspace_bytes(n_cells, block_size, n_batches, n_iter);
+  HZ_RET_IF(!ws || ws_bytes < need, HZ_ERR_WORKSPACE, "workspace too small: %zu < %zu", ws_bytes,
+            need);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const PermDims p = member_dims(n_cells, block_size, n_batches);
+  Carve c(ws);
+  PermBufs b;
+  ChunkTab t;
+  int* tmp = nullptr;
+  carve_perm(c, p, b, t, &tmp, n_iter, !member_fits(p));
+  int* err = c.take<int>(64);
+  HZ_TRY(fill<int>(err, 1, 0, s));
+  HZ_TRY(launch_block_order(p, seed, cats, b, t, tmp, nullptr, err, s, n_iter));
+  for (int y = 0; y < n_iter; ++y) {
+    const ChunkTab ty = tab_of(t, p, y);
+    hipLaunchKernelGGL(member_scatter_kernel, dim3(launch_grid_1d(p.maxch)), dim3(256), 0, s,
+                       ty.order, ty.seg_ch, ty.ch_start, ty.ch_len, ty.ch_seg,
+                       (long long)p.nblk * p.nb, p.nb, p.maxch,
+                       block_of_cell + (long long)y * n_cells,
+                       order_out ? order_out + (long long)y * n_cells : nullptr);
+    HZ_LAUNCHED("member_scatter");
+  }
+  int h_err = 0;
+  HZ_HIP(hipMemcpyAsync(&h_err, err, sizeof(int), hipMemcpyDeviceToHost, s));
+  HZ_HIP(hipStreamSynchronize(s));
+  HZ_RET_IF(h_err != 0, HZ_ERR_UNSUPPORTED,
+            "block order: a block-boundary bucket held more than %d keys", kBoundCap);
+  return HZ_OK;
 }
 
 int hz_init_clusters(const hz_dims* dims, const void* Z_norm, const void* Y0, const int32_t* cats,

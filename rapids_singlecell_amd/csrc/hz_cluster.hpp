@@ -28,8 +28,8 @@ constexpr int kZtStride = 68;  // padded row stride of the transposed Z tile in 
 struct Ctrl {
   int done;
   int n_obj;
-  int pad0;  // set when a bounded grid-barrier wait ran out
-  int pad1;
+  int pad0;       // set when a bounded grid-barrier wait ran out
+  int order_err;  // set when a block-order boundary bucket overflowed its key list
   double hist[4];
 };
 
@@ -669,11 +669,12 @@ __global__ __launch_bounds__(kBlock) void centroids_to_yt_kernel(const T* Y, T* 
 }
 
 // Control-block reset at the start of a clustering call.
-__global__ void ctrl_reset_kernel(Ctrl* c) {
+static __global__ void ctrl_reset_kernel(Ctrl* c) {
   if (threadIdx.x == 0) {
     c->done = 0;
     c->n_obj = 0;
     c->pad0 = 0;
+    c->order_err = 0;
     for (int i = 0; i < 4; ++i) c->hist[i] = 0.0;
   }
 }

@@ -51,7 +51,7 @@ __device__ __forceinline__ void split_bf16(const float (&x)[8], bf16x8& hi, bf16
 //   YT[c * Kp + k]           FP32, c-major (B operand of the f32 op)
 //   YH/YL[k * dP + c]        BF16 hi/lo, k-major, dP = round_up(d, 32)
 // One workgroup per centroid row (its L2 norm is computed in `yreduce`).
-__global__ void centroid_split_kernel(const float* __restrict__ YT, __bf16* __restrict__ YH,
+static __global__ void centroid_split_kernel(const float* __restrict__ YT, __bf16* __restrict__ YH,
                                       __bf16* __restrict__ YL, int K, int Kp, int d, int dP,
                                       const Ctrl* ctrl) {
   if (ctrl && ctrl->done) return;

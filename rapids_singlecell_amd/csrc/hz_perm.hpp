@@ -163,7 +163,7 @@ static inline ChunkTab tab_of(const ChunkTab& t, const PermDims& p, int i) {
 
 // Every kernel below handles iteration blockIdx.y of a batch: seed + y, and
 // buffers offset by y times their PermStride.
-__global__ void perm_keys_kernel(const int* __restrict__ done, int n, uint32_t seed,
+static __global__ void perm_keys_kernel(const int* __restrict__ done, int n, uint32_t seed,
                                  int shift, uint32_t* __restrict__ keys,
                                  int* __restrict__ bhist, PermStride st) {
   if (done && *done) return;
@@ -182,7 +182,7 @@ __global__ void perm_keys_kernel(const int* __restrict__ done, int n, uint32_t s
 // coalesced tiles of kScanThreads * kScanPer elements with a running carry.
 // out[0..n] (out[n] = total).  Optional second output `out2[0..n-1]`.
 // `lds` holds at least kScanThreads / 64 ints.
-__device__ void wg_exclusive_scan(const int* __restrict__ in, int* __restrict__ out,
+static __device__ void wg_exclusive_scan(const int* __restrict__ in, int* __restrict__ out,
                                   int* __restrict__ out2, long long n, int* lds) {
   constexpr int kWaves = kScanThreads / 64;
   constexpr int kTile = kScanThreads * kScanPer;
@@ -228,7 +228,7 @@ __device__ void wg_exclusive_scan(const int* __restrict__ in, int* __restrict__ 
   __syncthreads();
 }
 
-__global__ __launch_bounds__(kScanThreads) void perm_bucket_scan_kernel(
+static __global__ __launch_bounds__(kScanThreads) void perm_bucket_scan_kernel(
     const int* __restrict__ done, const int* __restrict__ bhist, int* __restrict__ bstart,
     int* __restrict__ bcur, int nbk, PermStride st) {
   __shared__ int lds[kScanThreads];
@@ -237,7 +237,7 @@ __global__ __launch_bounds__(kScanThreads) void perm_bucket_scan_kernel(
   wg_exclusive_scan(bhist + st.bk * y, bstart + st.bk1 * y, bcur + st.bk * y, nbk, lds);
 }
 
-__global__ void perm_scatter_kernel(const int* __restrict__ done, int n, int shift,
+static __global__ void perm_scatter_kernel(const int* __restrict__ done, int n, int shift,
                                     const uint32_t* __restrict__ keys, int* __restrict__ bcur,
                                     uint32_t* __restrict__ tk, int* __restrict__ ti,
                                     PermStride st) {
@@ -264,7 +264,7 @@ __global__ void perm_scatter_kernel(const int* __restrict__ done, int n, int shi
 // zeroed beforehand).
 constexpr int kRankTile = 512;
 constexpr int kRankLds = 2048;
-__global__ __launch_bounds__(256) void perm_rank_kernel(
+static __global__ __launch_bounds__(256) void perm_rank_kernel(
     const int* __restrict__ done, int n, int shift, const uint32_t* __restrict__ tk,
     const int* __restrict__ ti, const int* __restrict__ bstart, int* __restrict__ perm,
     const int* __restrict__ cats, int bs, int nb, int tpb, int* __restrict__ thist,
@@ -311,7 +311,7 @@ __global__ __launch_bounds__(256) void perm_rank_kernel(
 // Stable placement: cells of tile (block j, tile t) go to their batch segment
 // in key order.  One wave per tile; ranks come from ballots, so the result is
 // deterministic.
-__global__ __launch_bounds__(64) void group_place_kernel(
+static __global__ __launch_bounds__(64) void group_place_kernel(
     const int* __restrict__ done, const int* __restrict__ perm, const int* __restrict__ cats,
     int n, int bs, int nb, int tpb, const int* __restrict__ tbase, int* __restrict__ order,
     PermStride st) {
@@ -377,7 +377,7 @@ __global__ __launch_bounds__(64) void group_place_kernel(
 // Segment starts and first chunk of each segment from per-tile offsets (one
 // workgroup; chunk_fill_kernel then writes the chunk entries).
 // `tpb` = tiles per segment (1 for a plain segment-offset input).
-__global__ __launch_bounds__(kScanThreads) void chunk_table_kernel(
+static __global__ __launch_bounds__(kScanThreads) void chunk_table_kernel(
     const int* __restrict__ done, const int* __restrict__ thist, int* __restrict__ tbase,
     long long nseg, int tpb, int n, int chunk, int* __restrict__ seg_start,
     int* __restrict__ seg_ch, int* __restrict__ ch_start, int* __restrict__ ch_len,
@@ -407,7 +407,7 @@ __global__ __launch_bounds__(kScanThreads) void chunk_table_kernel(
 
 // One thread per chunk: its segment is the last s with seg_ch[s] <= c (empty
 // segments share their seg_ch with the next one), then start, length, segment.
-__global__ void chunk_fill_kernel(const int* __restrict__ done, long long nseg, int chunk,
+static __global__ void chunk_fill_kernel(const int* __restrict__ done, long long nseg, int chunk,
                                   const int* __restrict__ seg_start,
                                   const int* __restrict__ seg_ch, int* __restrict__ ch_start,
                                   int* __restrict__ ch_len, int* __restrict__ ch_seg,
@@ -459,7 +459,7 @@ __device__ __forceinline__ int bucket_of(const int* __restrict__ bstart, int nbk
 
 // bmark[b] = first boundary j (1-based) whose rank j*bs falls in bucket b
 // (bmark zeroed beforehand); bcnt zeroed.
-__global__ void bound_mark_kernel(const int* __restrict__ bstart, int* __restrict__ bmark,
+static __global__ void bound_mark_kernel(const int* __restrict__ bstart, int* __restrict__ bmark,
                                   int* __restrict__ bcnt, int nbk, int nblk, int bs,
                                   PermStride st) {
   const int y = blockIdx.y;
@@ -474,7 +474,7 @@ __global__ void bound_mark_kernel(const int* __restrict__ bstart, int* __restric
 }
 
 // keys of the marked buckets, appended to the list of their first boundary
-__global__ void bound_collect_kernel(const uint32_t* __restrict__ keys,
+static __global__ void bound_collect_kernel(const uint32_t* __restrict__ keys,
                                      const int* __restrict__ bmark, int* __restrict__ bcnt,
                                      uint32_t* __restrict__ blist, int n, int shift,
                                      PermStride st) {
@@ -494,11 +494,13 @@ __global__ void bound_collect_kernel(const uint32_t* __restrict__ keys,
 }
 
 // thr[j] = key of rank j*bs: the (j*bs - bstart[b])-th smallest key of its
-// bucket b (keys are distinct).  One workgroup per boundary.
-__global__ __launch_bounds__(256) void bound_select_kernel(
+// bucket b (keys are distinct).  One workgroup per boundary.  A bucket whose
+// keys did not all fit in its list (more than kBoundCap) sets *err and leaves
+// thr[j] undefined: the caller reports the order as failed.
+static __global__ __launch_bounds__(256) void bound_select_kernel(
     const int* __restrict__ bstart, const int* __restrict__ bmark, const int* __restrict__ bcnt,
     const uint32_t* __restrict__ blist, uint32_t* __restrict__ thr, int nbk, int bs,
-    PermStride st) {
+    int* __restrict__ err, PermStride st) {
   __shared__ uint32_t sk[kBoundCap];
   const int y = blockIdx.y;
   bstart += st.bk1 * y;
@@ -510,7 +512,11 @@ __global__ __launch_bounds__(256) void bound_select_kernel(
   const int P = j * bs;
   const int b = bucket_of(bstart, nbk, P);
   const int m = bmark[b];
-  const int cnt = min(bcnt[m], kBoundCap);
+  if (bcnt[m] > kBoundCap) {
+    if (threadIdx.x == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return;
+  }
+  const int cnt = bcnt[m];
   const int target = P - bstart[b];
   for (int t = threadIdx.x; t < cnt; t += blockDim.x) sk[t] = blist[(long long)m * kBoundCap + t];
   __syncthreads();
@@ -535,7 +541,7 @@ __device__ __forceinline__ int block_of(const uint32_t* __restrict__ sthr, int n
 
 // (segment, tile) histogram of the cells, tiles of T2 consecutive cells.
 // LDS: nseg counters, then nblk thresholds.
-__global__ __launch_bounds__(256) void member_count_kernel(
+static __global__ __launch_bounds__(256) void member_count_kernel(
     const uint32_t* __restrict__ keys, const uint32_t* __restrict__ thr,
     const int* __restrict__ cats, int n, int nblk, int nb, int tiles, int T2,
     int* __restrict__ thist, PermStride st) {
@@ -562,7 +568,7 @@ __global__ __launch_bounds__(256) void member_count_kernel(
 // Stable placement in cell order: wave w takes quarter w of the tile; its
 // first slot in each segment is the tile's segment offset plus the counts of
 // the waves before it.  LDS: 4 x nseg slots, then nblk thresholds.
-__global__ __launch_bounds__(256) void member_place_kernel(
+static __global__ __launch_bounds__(256) void member_place_kernel(
     const uint32_t* __restrict__ keys, const uint32_t* __restrict__ thr,
     const int* __restrict__ cats, int n, int nblk, int nb, int tiles, int T2,
     const int* __restrict__ tbase, int* __restrict__ order, PermStride st) {
@@ -655,7 +661,7 @@ static int launch_key_order(const PermDims& p, uint32_t seed, PermBufs& b, const
 // Block orders, grouping and chunk tables of I consecutive clustering
 // iterations (seeds seed .. seed+I-1) in one set of launches.
 static int launch_member_order(const PermDims& p, uint32_t seed, const int* cats, PermBufs& b,
-                               ChunkTab& t, int* nch_tmp, hipStream_t s, int I) {
+                               ChunkTab& t, int* nch_tmp, int* err, hipStream_t s, int I) {
   const int shift = 32 - p.nbk_bits;
   const int nbk = 1 << p.nbk_bits;
   const PermStride st = perm_stride(p);
@@ -678,7 +684,7 @@ static int launch_member_order(const PermDims& p, uint32_t seed, const int* cats
                        b.blist, p.n, shift, st);
     HZ_LAUNCHED("bound_collect");
     hipLaunchKernelGGL(bound_select_kernel, dim3(p.nblk - 1, I), dim3(256), 0, s, b.bstart,
-                       bmark, b.bcnt, b.blist, b.thr, nbk, p.bs, st);
+                       bmark, b.bcnt, b.blist, b.thr, nbk, p.bs, err, st);
     HZ_LAUNCHED("bound_select");
   }
   const int T2 = kMemberTile;
@@ -702,10 +708,13 @@ static int launch_member_order(const PermDims& p, uint32_t seed, const int* cats
   return HZ_OK;
 }
 
+// `err` (device int) is set when the membership path fails (see
+// bound_select_kernel); it must be cleared before and read after.
 static int launch_block_order(const PermDims& p, uint32_t seed, const int* cats, PermBufs& b,
-                              ChunkTab& t, int* nch_tmp, const int* done, hipStream_t s,
+                              ChunkTab& t, int* nch_tmp, const int* done, int* err, hipStream_t s,
                               int I = 1) {
-  if (!done && member_fits(p)) return launch_member_order(p, seed, cats, b, t, nch_tmp, s, I);
+  if (!done && member_fits(p))
+    return launch_member_order(p, seed, cats, b, t, nch_tmp, err, s, I);
   HZ_TRY(launch_key_order(p, seed, b, done, s, cats, I));
   const long long nseg = (long long)p.nblk * p.nb;
   const size_t lds = (size_t)p.nb * sizeof(int);
@@ -725,9 +734,31 @@ static int launch_block_order(const PermDims& p, uint32_t seed, const int* cats,
   return HZ_OK;
 }
 
+// block_of_cell[cell] = block of the cell's segment; order_out (optional) =
+// the grouped order itself.  One thread per chunk.
+static __global__ void member_scatter_kernel(const int* __restrict__ order,
+                                             const int* __restrict__ seg_ch,
+                                             const int* __restrict__ ch_start,
+                                             const int* __restrict__ ch_len,
+                                             const int* __restrict__ ch_seg, long long nseg, int nb,
+                                             long long maxch, int* __restrict__ block_of_cell,
+                                             int* __restrict__ order_out) {
+  const long long total = seg_ch[nseg];
+  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < total && c < maxch;
+       c += (long long)gridDim.x * blockDim.x) {
+    const int p0 = ch_start[c], len = ch_len[c];
+    const int blk = ch_seg[c] / nb;
+    for (int q = 0; q < len; ++q) {
+      const int cell = order[p0 + q];
+      block_of_cell[cell] = blk;
+      if (order_out) order_out[p0 + q] = cell;
+    }
+  }
+}
+
 // Chunk table for a static segmentation (cells grouped by batch):
 // seg_start = cat_offsets (copied), order = cell_indices (copied).
-__global__ void copy_int_kernel(const int* __restrict__ src, int* __restrict__ dst, long long n) {
+static __global__ void copy_int_kernel(const int* __restrict__ src, int* __restrict__ dst, long long n) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
     dst[i] = src[i];

@@ -123,7 +123,7 @@ __global__ __launch_bounds__(kBlock) void insum_kernel(InSumArgs a) {
 // [nlo + w*per, min(nhi, nlo + (w+1)*per)) of block j and adds to every batch
 // from its first chunk's to its last chunk's, in replica w % kSweepRep (the
 // same rule as the sweep's flush).  One workgroup per block.
-__global__ __launch_bounds__(kBlock) void sweep_count_kernel(const int* __restrict__ done,
+static __global__ __launch_bounds__(kBlock) void sweep_count_kernel(const int* __restrict__ done,
                                                              const int* __restrict__ seg_ch,
                                                              const int* __restrict__ ch_seg,
                                                              int* __restrict__ cnt, int B, int G) {

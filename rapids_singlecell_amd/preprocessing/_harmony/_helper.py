@@ -41,10 +41,11 @@ def _get_theta_array(theta, n_batches: int, dtype) -> np.ndarray:
     """Theta broadcast to one value per batch.
 
     preprocessing/_harmony/_helper.py:277-307 of the reference (scalars are
-    broadcast; arrays must have ``n_batches`` elements).
+    broadcast; arrays must have ``n_batches`` elements).  ``bool`` is an
+    ``int`` there, so ``theta=True`` means 1.0.
     """
     dtype = np.dtype(dtype)
-    if isinstance(theta, (float, int)) and not isinstance(theta, bool):
+    if isinstance(theta, (float, int)):
         return np.ones(n_batches, dtype=dtype) * float(theta)
     if isinstance(theta, (list, np.ndarray)):
         arr = np.asarray(theta, dtype=dtype)

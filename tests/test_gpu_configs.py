@@ -1,0 +1,153 @@
+"""Parity at the benchmark configurations, full size, against the oracle (GPU).
+
+C1: 10,000 x 50 PCs, 10 Gaussian cell types, 3 uniform batches, K = 50,
+harmony1 with ridge_lambda = 1, 10 Harmony x 20 clustering iterations.
+C2: 200,000 x 50 PCs, 30 types, 8 Dirichlet(1)-sized batches, K = 100,
+harmony1, ridge_lambda = 1 (2 Harmony x 20 clustering iterations at fixed
+counts, so that the oracle finishes in well under a minute).
+
+Bounds (relative Frobenius error of Z_corr):
+  FP64                       <= 1e-8 of the FP64 oracle, identical iteration counts
+  plain FP32 ("native")      <= 1e-4 of the oracle run in FP64 and in FP32
+  split-BF16                 <= 1e-3 of the FP64 oracle
+Every test prints the error it observed.  Initial centroids are injected
+(K random rows of Z_norm): the reference's cuML k-means cannot run here.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import harmony_oracle as ho
+from rapids_singlecell_amd.datasets import synthetic_pcs
+from rapids_singlecell_amd.preprocessing._harmony import harmonize
+
+pytestmark = pytest.mark.gpu
+
+NEG_INF = float("-inf")
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+class Config:
+    def __init__(self, n, n_types, B, K, dirichlet, seed=0):
+        X, obs = synthetic_pcs(n, 50, n_types, B, dirichlet_batches=dirichlet, seed=seed)
+        self.X32 = X
+        self.X64 = X.astype(np.float64)
+        self.obs = obs
+        self.codes, cats, _ = ho.batch_codes(obs, "batch")
+        self.B = len(cats)
+        self.K = K
+        Z = ho.l2_normalize(self.X64)
+        self.Y0 = Z[np.random.default_rng(seed + 7).choice(n, K, replace=False)].copy()
+        self._oracle = {}
+
+    def params(self, **kw):
+        base = dict(n_clusters=self.K, sigma=0.1, theta=2.0, ridge_lambda=1.0,
+                    stabilized_penalty=False, dynamic_lambda=False, block_proportion=0.05,
+                    correction_method="batched", random_state=0)
+        base.update(kw)
+        return base
+
+    def oracle(self, dtype, **kw):
+        key = (np.dtype(dtype).name, tuple(sorted(kw.items())))
+        if key not in self._oracle:
+            X = self.X64 if dtype == np.float64 else self.X32
+            rec = {}
+            Z = ho.harmonize(X, self.codes, self.B, Y0=self.Y0.astype(dtype), record=rec,
+                             **self.params(**kw))
+            self._oracle[key] = (Z, rec)
+        return self._oracle[key]
+
+    def gpu(self, dtype, precision="native", **kw):
+        dev = torch.device("cuda", 0)
+        X = self.X64 if dtype == np.float64 else self.X32
+        rec = {}
+        Z = harmonize(torch.as_tensor(X, device=dev), self.obs, "batch",
+                      init_centroids=self.Y0.astype(dtype), precision=precision, record=rec,
+                      **self.params(**kw))
+        return Z.cpu().numpy(), rec
+
+
+@pytest.fixture(scope="module")
+def c1():
+    return Config(10_000, 10, 3, 50, dirichlet=False)
+
+
+@pytest.fixture(scope="module")
+def c2():
+    return Config(200_000, 30, 8, 100, dirichlet=True)
+
+
+C1_RULES = dict(max_iter_harmony=10, max_iter_clustering=20)
+C1_FIXED = dict(max_iter_harmony=10, max_iter_clustering=20, tol_harmony=NEG_INF,
+                tol_clustering=NEG_INF)
+C2_FIXED = dict(max_iter_harmony=2, max_iter_clustering=20, tol_harmony=NEG_INF,
+                tol_clustering=NEG_INF)
+
+
+def test_c1_fp64_reference_rules(device, c1):
+    """C1 with the reference's convergence rules: same iteration counts."""
+    Zr, rr = c1.oracle(np.float64, **C1_RULES)
+    Zg, rg = c1.gpu(np.float64, **C1_RULES)
+    e = rel(Zg, Zr)
+    print(f"\nC1 fp64 (reference rules): clustering iters gpu={rg['clustering_iters']} "
+          f"oracle={rr['clustering_iters']}, rel err Z_corr = {e:.3e}", flush=True)
+    assert rg["clustering_iters"] == rr["clustering_iters"]
+    np.testing.assert_allclose(rg["objectives"], rr["objectives"], rtol=1e-9)
+    assert e < 1e-8
+
+
+def test_c1_fp32_fixed_iters(device, c1):
+    Z64, _ = c1.oracle(np.float64, **C1_FIXED)
+    Z32, _ = c1.oracle(np.float32, **C1_FIXED)
+    Zg, rg = c1.gpu(np.float32, **C1_FIXED)
+    e64, e32 = rel(Zg, Z64), rel(Zg, Z32)
+    print(f"\nC1 fp32 10x20 iters: rel err Z_corr vs FP64 oracle = {e64:.3e}, "
+          f"vs FP32 oracle = {e32:.3e}", flush=True)
+    assert rg["clustering_iters"] == [20] * 10
+    assert e64 < 1e-4 and e32 < 1e-4
+
+
+def test_c1_split_bf16_fixed_iters(device, c1):
+    Z64, _ = c1.oracle(np.float64, **C1_FIXED)
+    Zg, _ = c1.gpu(np.float32, precision="split_bf16", **C1_FIXED)
+    e = rel(Zg, Z64)
+    print(f"\nC1 split-BF16 10x20 iters: rel err Z_corr vs FP64 oracle = {e:.3e}", flush=True)
+    assert e < 1e-3
+
+
+def test_c2_fp64_fixed_iters(device, c2):
+    Zr, rr = c2.oracle(np.float64, **C2_FIXED)
+    Zg, rg = c2.gpu(np.float64, **C2_FIXED)
+    e = rel(Zg, Zr)
+    print(f"\nC2 fp64 2x20 iters: rel err Z_corr = {e:.3e}", flush=True)
+    np.testing.assert_allclose(rg["objectives"], rr["objectives"], rtol=1e-9)
+    assert e < 1e-8
+
+
+def test_c2_fp32_fixed_iters(device, c2):
+    Z64, r64 = c2.oracle(np.float64, **C2_FIXED)
+    Z32, _ = c2.oracle(np.float32, **C2_FIXED)
+    Zg, rg = c2.gpu(np.float32, **C2_FIXED)
+    e64, e32 = rel(Zg, Z64), rel(Zg, Z32)
+    eR = rel(rg["R"].cpu().numpy(), r64["R"])
+    print(f"\nC2 fp32 2x20 iters: rel err Z_corr vs FP64 oracle = {e64:.3e}, vs FP32 oracle = "
+          f"{e32:.3e}; R vs FP64 oracle = {eR:.3e}", flush=True)
+    assert e64 < 1e-4 and e32 < 1e-4
+
+
+def test_c2_split_bf16_fixed_iters(device, c2):
+    Z64, r64 = c2.oracle(np.float64, **C2_FIXED)
+    Zg, rg = c2.gpu(np.float32, precision="split_bf16", **C2_FIXED)
+    e = rel(Zg, Z64)
+    eR = rel(rg["R"].cpu().numpy(), r64["R"])
+    print(f"\nC2 split-BF16 2x20 iters: rel err Z_corr vs FP64 oracle = {e:.3e}; "
+          f"R = {eR:.3e}", flush=True)
+    assert e < 1e-3

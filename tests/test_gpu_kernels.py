@@ -189,3 +189,55 @@ def test_objective(device, dtype, n_cells, n_clusters, n_batches, stabilized):
                        E.astype(np.float64), theta.astype(np.float64), 0.1, stabilized)
     rtol = 1e-5 if dtype == np.float32 else 1e-10
     np.testing.assert_allclose(res.value, exp, rtol=rtol)
+
+
+def _membership(device, n, bs, cats, seed, n_iter):
+    lib = _native.load()
+    B = int(cats.max()) + 1
+    nb = lib.hz_block_membership_workspace_bytes(n, bs, B, n_iter)
+    assert nb > 0
+    ws = torch.empty(nb, dtype=torch.uint8, device=device)
+    c = _t(cats.astype(np.int32), device)
+    blk = torch.full((n_iter, n), -1, dtype=torch.int32, device=device)
+    order = torch.full((n_iter, n), -1, dtype=torch.int32, device=device)
+    _native.check(lib.hz_block_membership(n, bs, B, _p(c), seed & 0xFFFFFFFF, n_iter, _p(blk),
+                                          _p(order), _p(ws), nb, _s(device)), "membership")
+    return blk.cpu().numpy(), order.cpu().numpy()
+
+
+def _check_membership(n, bs, cats, seed, blk, order):
+    B = int(cats.max()) + 1
+    for y in range(blk.shape[0]):
+        perm = ho.block_order(n, (seed + y) & 0xFFFFFFFF)
+        rank = np.empty(n, np.int64)
+        rank[perm] = np.arange(n)
+        # bit-exact block of every cell: rank in the key-sorted order // block size
+        np.testing.assert_array_equal(blk[y], rank // bs, err_msg=f"iteration {y}")
+        # the grouped order is a permutation sorted by (block, batch)
+        o = order[y]
+        np.testing.assert_array_equal(np.sort(o), np.arange(n))
+        key = (rank[o] // bs) * B + cats[o]
+        assert np.all(np.diff(key) >= 0)
+
+
+@pytest.mark.parametrize("n,B", [(200_000, 8), (1_300_000, 16)])
+@pytest.mark.parametrize("seed", [0, 1000003, 0xFFFFFFF8])
+def test_block_membership_bit_exact(device, n, B, seed):
+    """The block membership the clustering loop builds (boundary keys +
+    counting sort, 16 iterations per set of launches, seeds wrapping at 2^32)
+    equals the reference's key-sorted order cut into blocks
+    (clustering.cu:292-319) bit for bit."""
+    rng = np.random.default_rng(seed % 1000)
+    cats = rng.integers(0, B, n).astype(np.int32)
+    bs = int(n * 0.05)
+    blk, order = _membership(device, n, bs, cats, seed, 16)
+    _check_membership(n, bs, cats, seed, blk, order)
+
+
+def test_block_membership_key_order_fallback(device):
+    """100 blocks x 41 batches = 4100 segments do not fit the membership
+    kernels' LDS: the full key ranking is used instead; same membership."""
+    n, B = 5000, 41
+    cats = (np.arange(n) % B).astype(np.int32)
+    blk, order = _membership(device, n, 50, cats, 77, 3)
+    _check_membership(n, 50, cats, 77, blk, order)

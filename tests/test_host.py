@@ -138,6 +138,14 @@ def test_theta_array_matches_oracle():
         _get_theta_array("2", 3, np.float64)
 
 
+def test_theta_bool_is_an_int():
+    """The reference tests ``isinstance(theta, float | int)`` (_helper.py:286),
+    which a bool passes: True broadcasts as 1.0 and False as 0.0."""
+    np.testing.assert_array_equal(_get_theta_array(True, 3, np.float32), np.ones(3, np.float32))
+    np.testing.assert_array_equal(_get_theta_array(False, 2, np.float64), np.zeros(2))
+    np.testing.assert_array_equal(ho.theta_array(True, 3, np.float32), np.ones(3, np.float32))
+
+
 def test_category_index_mapping_matches_oracle():
     cats = np.random.default_rng(1).integers(0, 5, 1000).astype(np.int32)
     off, idx = _create_category_index_mapping(cats, 5)
