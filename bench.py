@@ -1,16 +1,24 @@
 """Harmony throughput benchmark on one MI355X.
 
-Workload (default): plain FP32 Harmony (harmony1, ridge_lambda=1) on a
-synthetic PCA embedding of N=200,000 cells x 50 PCs from a 30-type Gaussian
-mixture with 8 Dirichlet(1)-sized batches and per-batch shifts ~N(0,1);
-K=100, theta=2, sigma=0.1, block_proportion=0.05, 20 clustering iterations per
-Harmony iteration (early stop disabled).  One step = one Harmony iteration
-(clustering + ridge correction) on device-resident state; the metric is
-cells x Harmony iterations per second.
+Default workload: 1,300,000 cells x 50 PCs from a 50-type Gaussian mixture
+with 16 uniform batches (one-hot Phi, B = 16) and per-batch shifts
+~N(0, 0.5^2); K = 100, theta = 2, sigma = 0.1, harmony1 with lambda = 1,
+block_proportion = 0.05, 20 clustering iterations per Harmony iteration (early
+stop off).  One step = one Harmony iteration (clustering + ridge correction)
+on device-resident state; the metric is cells x Harmony iterations per second.
 
-Also reported: the roofline of the dominant kernel (algorithmic bytes / live
-HIP-event launch time vs 8 TB/s) and a CPU baseline (the NumPy FP64 oracle on
-a bounded sample of the same workload, timed on this host).
+One run measures, on the same workload:
+  * plain FP32 and split-BF16 throughput (timed steps), and the batched form
+    (several instances of the workload in one multi-job clustering loop);
+    ``value`` is the split-BF16 batched figure, the others sit beside it;
+  * the relative error of Z_corr, R and Y of both precisions after two Harmony
+    iterations against the FP64 path from the same initial centroids;
+  * p50/p99 job latency of a short served leg (Poisson job stream);
+  * the roofline of the dominant kernel (algorithmic bytes / live HIP-event
+    launch time vs 8 TB/s);
+  * a CPU baseline: the NumPy FP64 oracle on this host, at the full N for two
+    clustering iterations (median of 3) plus the correction on a sample,
+    extrapolated to one Harmony iteration.
 """
 
 from __future__ import annotations
@@ -32,20 +40,36 @@ HBM_PEAK_GBS = 8000.0
 COPY_BYTES = 256 << 20
 
 WORKLOADS = {
-    # name: (n_cells, n_pcs, n_types, n_batches, n_clusters)
-    "c2": (200_000, 50, 30, 8, 100),
-    "c3": (1_300_000, 50, 50, 16, 100),
+    # name: (n_cells, n_pcs, n_types, n_batches, n_clusters, Dirichlet batch sizes, shift sd)
+    "c2": (200_000, 50, 30, 8, 100, True, 1.0),
+    "c3": (1_300_000, 50, 50, 16, 100, False, 0.5),
 }
 
 
 def make_data(name: str, seed: int = 0):
     from rapids_singlecell_amd.datasets import synthetic_pcs
 
-    n, d, types, B, K = WORKLOADS[name]
-    X, obs = synthetic_pcs(n, d, types, B, dirichlet_batches=True, seed=seed,
-                           shift_scale=1.0 if name == "c2" else 0.5)
+    n, d, types, B, K, dirichlet, shift = WORKLOADS[name]
+    X, obs = synthetic_pcs(n, d, types, B, dirichlet_batches=dirichlet, seed=seed,
+                           shift_scale=shift)
     codes = np.asarray(obs["batch"].cat.codes.values, dtype=np.int32)
     return X, codes, B, K
+
+
+def workload_text(name: str) -> str:
+    n, d, types, B, K, dirichlet, shift = WORKLOADS[name]
+    sizes = "Dirichlet(1)-sized" if dirichlet else "uniform"
+    return (f"harmony1 N={n} d={d} K={K} B={B} (one-hot, {sizes} batches, shifts sd {shift}) "
+            f"{types}-type mixture, theta=2 sigma=0.1 lambda=1 block_proportion=0.05, "
+            f"20 clustering iters per Harmony iter")
+
+
+def initial_centroids(X, K, seed=7):
+    """K random rows of the L2-normalised input (FP64): the same initial
+    centroids for every precision of one comparison."""
+    Xd = X.astype(np.float64)
+    Z = Xd / np.maximum(np.linalg.norm(Xd, axis=1, keepdims=True), 1e-12)
+    return Z[np.random.default_rng(seed).choice(len(X), K, replace=False)]
 
 
 def build_job(X, codes, B, K, precision, stream=None):
@@ -54,9 +78,10 @@ def build_job(X, codes, B, K, precision, stream=None):
     from rapids_singlecell_amd.preprocessing._harmony._job import HarmonyJob
 
     dev = torch.device("cuda", 0)
-    N_b = np.bincount(codes, minlength=B).astype(np.float32)
+    dt = X.dtype
+    N_b = np.bincount(codes, minlength=B).astype(dt)
     job = HarmonyJob(torch.as_tensor(X, device=dev), codes, B, n_clusters=K,
-                     theta_np=np.full(B, 2.0, np.float32), N_b_np=N_b, sigma=0.1,
+                     theta_np=np.full(B, 2.0, dt), N_b_np=N_b, sigma=0.1,
                      block_proportion=0.05, max_iter_clustering=20,
                      tol_clustering=float("-inf"), ridge_lambda=1.0, stabilized_penalty=False,
                      dynamic_lambda=False, correction_method="batched", random_state=0,
@@ -129,9 +154,17 @@ def roofline(job, first_iter: int, steps: int, esz: int, precision: str) -> tupl
     return roof, kernels
 
 
-def cpu_baseline(X, codes, B, K, n_sample: int) -> dict:
-    """NumPy FP64 oracle: 1 Harmony iteration (20 clustering iterations) on the
-    first `n_sample` cells of the same workload."""
+def cpu_baseline(X, codes, B, K, reps: int = 3, n_clust: int = 2,
+                 corr_sample: int = 25_000) -> dict:
+    """NumPy FP64 oracle on this host, extrapolated to one Harmony iteration.
+
+    Clustering: ``n_clust`` clustering iterations over all N cells (median of
+    ``reps`` runs), scaled to 20.  Correction: the oracle's batched correction
+    (whose apply step builds a K x n x d temporary) on the first
+    ``corr_sample`` cells (median of ``reps``), scaled linearly to N.
+    """
+    import statistics
+
     from oracle import harmony_oracle as ho
 
     try:
@@ -140,28 +173,55 @@ def cpu_baseline(X, codes, B, K, n_sample: int) -> dict:
         blas_threads = max((i.get("num_threads", 1) for i in threadpool_info()), default=1)
     except Exception:  # noqa: BLE001
         blas_threads = None
-    Xs = X[:n_sample].astype(np.float64)
-    cs = codes[:n_sample]
-    Z = ho.l2_normalize(Xs)
-    Y0 = Z[np.random.default_rng(0).choice(n_sample, K, replace=False)]
-    t0 = time.perf_counter()
-    ho.harmonize(Xs, cs, B, Y0=Y0, n_clusters=K, max_iter_harmony=1, max_iter_clustering=20,
-                 tol_harmony=float("-inf"), tol_clustering=float("-inf"),
-                 stabilized_penalty=False, dynamic_lambda=False, ridge_lambda=1.0,
-                 random_state=0)
-    dt = time.perf_counter() - t0
+    n = X.shape[0]
+    Xd = X.astype(np.float64)
+    cats = np.asarray(codes, dtype=np.int32)
+    N_b = np.bincount(cats, minlength=B).astype(np.float64)
+    Pr_b = (N_b / n).reshape(-1, 1)
+    theta = np.full(B, 2.0)
+    Z = ho.l2_normalize(Xd)
+    Y0 = initial_centroids(X, K)
+    R0, E0, O0, _ = ho.init_clusters(Z, Y0, sigma=0.1, Pr_b=Pr_b, theta=theta, cats=cats,
+                                     n_batches=B, stabilized=False)
+    tc = []
+    for _ in range(reps):
+        R, E, O = R0.copy(), E0.copy(), O0.copy()
+        t0 = time.perf_counter()
+        ho.clustering_loop(Z, R, E, O, Pr_b, cats, theta, sigma=0.1, tol=float("-inf"),
+                           max_iter=n_clust, block_size=int(n * 0.05), seed=0, stabilized=False)
+        tc.append(time.perf_counter() - t0)
+    m = min(corr_sample, n)
+    lam = ho.compute_lambda_kb(E, O, N_b, alpha=0.2, threshold=None, ridge_lambda=1.0,
+                               dynamic_lambda=False)
+    tk = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        ho.correction_batched(Xd[:m], R[:m], O, lam, cats[:m], B)
+        tk.append(time.perf_counter() - t0)
+    t_clust = statistics.median(tc)
+    t_corr = statistics.median(tk)
+    t_iter = t_clust * (20 / n_clust) + t_corr * (n / m)
     try:
         aff = len(os.sched_getaffinity(0))
     except Exception:  # noqa: BLE001
         aff = os.cpu_count()
-    return {"value": round(n_sample / dt, 1), "unit": "cells·iters/s",
+    return {"value": round(n / t_iter, 1), "unit": "cells·iters/s",
             "cores": blas_threads or aff, "kind": "port",
-            "sample": f"oracle FP64 harmonize, first {n_sample} cells of the workload, "
-                      f"1 Harmony iter x 20 clustering iters, {dt:.1f} s "
-                      f"(os.cpu_count={os.cpu_count()}, affinity={aff}, blas_threads={blas_threads})"}
+            "sample": f"oracle FP64 (NumPy): {n_clust} clustering iterations over all {n} cells, "
+                      f"median of {reps} = {t_clust:.2f} s, scaled x{20 / n_clust:g}; batched "
+                      f"correction on the first {m} cells, median of {reps} = {t_corr:.3f} s, "
+                      f"scaled x{n / m:.1f}; one Harmony iteration = {t_iter:.1f} s "
+                      f"(os.cpu_count={os.cpu_count()}, affinity={aff}, "
+                      f"blas_threads={blas_threads})"}
 
 
 def served(args) -> int:
+    out = run_served(args, args.served_duration)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def run_served(args, duration: float) -> dict:
     """Served form: a Poisson stream of integration jobs through HarmonyServer.
 
     Job shapes: N log-uniform in [5k, 500k], d in {30, 50, 100}, B uniform in
@@ -197,7 +257,7 @@ def served(args) -> int:
         futs, t = [], 0.0
         while True:
             t += rng.exponential(1.0 / args.served_rate)
-            if t > args.served_duration:
+            if t > duration:
                 break
             now = time.perf_counter() - t0
             if t > now:
@@ -225,7 +285,7 @@ def served(args) -> int:
         "latency": latency_summary(lats),
         "jobs": len(res),
         "offered_rate_jobs_per_s": args.served_rate,
-        "duration_s": args.served_duration,
+        "duration_s": duration,
         "wall_s": round(wall, 2),
         "slots": args.served_slots,
         "server_stats": stats,
@@ -234,8 +294,7 @@ def served(args) -> int:
         "data": "synthetic Gaussian-mixture PCA embeddings (pool of "
                 f"{len(pool)} shapes drawn from the served-form distribution)",
     }
-    print(json.dumps(out), flush=True)
-    return 0
+    return out
 
 
 def batched(args) -> int:
@@ -330,16 +389,87 @@ def end_to_end(args) -> int:
     return 0
 
 
+def throughput_leg(X, codes, B, K, precision: str, warmup: int, steps: int,
+                   with_roofline: bool = True) -> dict:
+    """Time ``steps`` Harmony iterations of one job after ``warmup`` untimed ones
+    (inputs resident in HBM), then two more under live HIP-event timing."""
+    import torch
+
+    prec = {"native": 0, "split_bf16": 1}[precision]
+    job = build_job(X, codes, B, K, prec)
+    job.init()
+    torch.cuda.synchronize()
+    it = 0
+    for _ in range(warmup):
+        job.step(it)
+        it += 1
+    torch.cuda.synchronize()
+    dt = time_steps(job, it, steps)
+    it += steps
+    n = X.shape[0]
+    leg = {"precision": precision, "value": round(n * steps / dt, 1),
+           "ms_per_step": round(1e3 * dt / steps, 3), "steps": steps}
+    print(f"[bench] {precision}: {steps} steps in {dt:.3f} s -> {leg['value']:.4g} cells·iters/s",
+          file=sys.stderr, flush=True)
+    if with_roofline:
+        leg["roofline"], leg["kernels"] = roofline(job, it, 2, X.dtype.itemsize, precision)
+    del job
+    torch.cuda.empty_cache()
+    return leg
+
+
+def _fixed_run(X, codes, B, K, Y0, precision: int, iters: int):
+    """``iters`` Harmony iterations from the centroids Y0; returns the final
+    Z_corr, R and Y = L2(R^T Z_norm) as FP64 device tensors."""
+    import torch
+
+    job = build_job(X, codes, B, K, precision)
+    job.init(torch.as_tensor(Y0))
+    for i in range(iters):
+        job.step(i)
+    Z = job.Z_hat.double()
+    R = job.R.double()
+    Y = R.T @ job.Z_norm.double()
+    Y = Y / torch.linalg.norm(Y, dim=1, keepdim=True).clamp_min(1e-12)
+    del job
+    return Z, R, Y
+
+
+def error_leg(X, codes, B, K, iters: int = 2) -> dict:
+    """Relative Frobenius error of Z_corr, R and Y of the FP32 storage paths
+    (plain FP32 and split-BF16) against the FP64 path, all from the same
+    initial centroids, after ``iters`` Harmony iterations.  The FP64 path
+    equals the FP64 oracle within 1e-8 (tests at 10k and 200k cells)."""
+    import torch
+
+    Y0 = initial_centroids(X, K)
+    ref = _fixed_run(X.astype(np.float64), codes, B, K, Y0, 0, iters)
+    out = {"reference": f"FP64 GPU path, {iters} Harmony iterations x 20 clustering iterations, "
+                        "same initial centroids"}
+    for name, prec in (("native", 0), ("split_bf16", 1)):
+        got = _fixed_run(X, codes, B, K, Y0.astype(np.float32), prec, iters)
+        out[name] = {k: float(torch.linalg.norm(g - r) / torch.linalg.norm(r))
+                     for k, g, r in zip(("Z_corr", "R", "Y"), got, ref)}
+        del got
+    del ref
+    torch.cuda.empty_cache()
+    print(f"[bench] errors vs FP64: {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
-    ap.add_argument("--precision", default="native", choices=["native", "split_bf16"])
+    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default="split_bf16", choices=["native", "split_bf16"],
+                    help="precision of the --served/--batched/--end-to-end/--pmc-run modes")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=50_000)
+    ap.add_argument("--no-errors", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--served-leg", type=float, default=12.0,
+                    help="seconds of the served leg of the default run (0 = skip)")
     ap.add_argument("--pmc-run", action="store_true",
                     help="short run for hardware-counter passes: a 256 MiB device copy "
                          "(calibration kernel of known size) then 1 Harmony iteration")
@@ -379,15 +509,14 @@ def main() -> int:
     tune = {k: int(v) for k, v in (t.split("=", 1) for t in args.tune)}
     if tune:
         _native.set_tuning(**tune)
-    prec = {"native": 0, "split_bf16": 1}[args.precision]
     X, codes, B, K = make_data(args.workload)
     n, d = X.shape
-    print(f"[bench] workload {args.workload}: N={n} d={d} K={K} B={B} precision={args.precision}",
-          file=sys.stderr, flush=True)
-    job = build_job(X, codes, B, K, prec)
-    job.init()
-    torch.cuda.synchronize()
+    print(f"[bench] workload {args.workload}: N={n} d={d} K={K} B={B}", file=sys.stderr,
+          flush=True)
     if args.pmc_run:
+        job = build_job(X, codes, B, K, {"native": 0, "split_bf16": 1}[args.precision])
+        job.init()
+        torch.cuda.synchronize()
         # calibration: a plain copy reads and writes exactly COPY_BYTES
         src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
@@ -399,50 +528,54 @@ def main() -> int:
         print(json.dumps({"pmc_run": True, "copy_bytes": COPY_BYTES, "workload": args.workload,
                           "precision": args.precision}), flush=True)
         return 0
-    it = 0
-    for _ in range(args.warmup):
-        job.step(it)
-        it += 1
-    torch.cuda.synchronize()
-    print("[bench] warmup done", file=sys.stderr, flush=True)
-    dt = time_steps(job, it, args.steps)
-    it += args.steps
-    value = n * args.steps / dt
-    print(f"[bench] timed {args.steps} steps in {dt:.3f} s", file=sys.stderr, flush=True)
-    roof, kernels = (None, None)
-    if not args.no_roofline:
-        roof, kernels = roofline(job, it, 2, X.dtype.itemsize, args.precision)
-        it += 2
+
+    roof_on = not args.no_roofline
+    legs = {p: throughput_leg(X, codes, B, K, p, args.warmup, args.steps, roof_on)
+            for p in ("native", "split_bf16")}
+    head = legs["split_bf16"]
+    errors = None if args.no_errors else error_leg(X, codes, B, K)
+    served_out = None
+    if args.served_leg > 0:
+        print("[bench] served leg ...", file=sys.stderr, flush=True)
+        sv = run_served(args, args.served_leg)
+        served_out = {k: sv[k] for k in ("value", "latency", "jobs", "offered_rate_jobs_per_s",
+                                         "duration_s", "wall_s", "slots", "n_failures")}
     cpu = None
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(X, codes, B, K, args.cpu_sample)
+        cpu = cpu_baseline(X, codes, B, K)
     out = {
         "metric": METRIC,
-        "value": round(value, 1),
+        "value": head["value"],
         "unit": "cells·iters/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * dt / args.steps, 3),
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic: Gaussian-mixture PCA embedding with Dirichlet batch sizes and batch shifts",
+        "dtype": "f32 storage, split-bf16 products, f32 accumulation",
+        "data": "synthetic: Gaussian-mixture PCA embedding with per-batch shifts (no datasets offline)",
         "config": {
-            "workload": f"harmony1 N={n} d={d} K={K} B={B} theta=2 sigma=0.1 lambda=1 "
-                        f"block_proportion=0.05, 20 clustering iters per Harmony iter",
-            "precision": args.precision,
+            "workload": workload_text(args.workload),
+            "value_leg": "split_bf16, one job",
             "tuning": _native.get_tuning(),
             "n_cells": n, "n_pcs": d, "n_clusters": K, "n_batches": B,
         },
-        "roofline": roof,
+        "fp32": {k: legs["native"][k] for k in ("value", "ms_per_step")},
+        "split_bf16_vs_fp32": round(head["value"] / legs["native"]["value"], 3),
+        "rel_err_vs_fp64": errors,
+        "latency": served_out["latency"] if served_out else None,
+        "served": served_out,
+        "roofline": head.get("roofline"),
+        "roofline_fp32": legs["native"].get("roofline"),
         "cpu_baseline": cpu,
-        "kernels": kernels,
+        "kernels": head.get("kernels"),
+        "kernels_fp32": legs["native"].get("kernels"),
     }
     if cpu:
-        out["speedup_vs_cpu"] = round(value / cpu["value"], 1)
+        out["fp32_vs_cpu"] = round(legs["native"]["value"] / cpu["value"], 1)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -45,7 +45,20 @@ extern "C" {
 #define HZ_CORR_FAST 1
 #define HZ_CORR_ORIGINAL 2
 
-/* Problem shape shared by the job-level entry points. */
+/* Problem shape shared by the job-level entry points.
+ *
+ * One covariate (the reference's case): n_batches levels, cats[i] is the
+ * level of cell i, and O, E, Pr_b, theta, N_b and lambda have n_batches rows.
+ *
+ * Several covariates (stacked one-hot Phi; no counterpart in the reference,
+ * which joins several keys into one categorical): every cell has one level
+ * of each of n_covariates covariates.  Cells with the same levels form a
+ * group; n_batches is then the number of groups, cats[i] the group of cell i
+ * (cat_offsets / cell_indices group the cells by group), and group_levels
+ * (device, n_batches x n_covariates int32) holds each group's level ids in
+ * 0 .. n_levels-1 (covariate c's levels numbered after those of covariates
+ * 0 .. c-1; n_levels0 = levels of covariate 0).  O, E, Pr_b, theta, N_b and
+ * lambda then have n_levels rows. */
 typedef struct hz_dims {
   int n_cells;
   int n_pcs;
@@ -54,10 +67,23 @@ typedef struct hz_dims {
   int block_size; /* int(n_cells * block_proportion), >= 1 */
   int dtype;      /* HZ_F32 or HZ_F64 */
   int precision;  /* HZ_PREC_* */
+  int n_covariates;              /* 0 or 1: one covariate; >= 2: stacked one-hot */
+  int n_levels;                  /* n_covariates >= 2: total levels */
+  int n_levels0;                 /* n_covariates >= 2: levels of covariate 0 */
+  const int32_t* group_levels;   /* n_covariates >= 2: n_batches x n_covariates */
 } hz_dims;
 
 int hz_version(void);
 const char* hz_last_error(void);
+
+/* Releases what the library created for the calling thread: the side stream
+ * (and its events) that builds block orders beside the clustering loop, and
+ * the pinned ring of convergence flags, on every device the thread used.  It
+ * waits for their pending work first.  Call it before a thread that ran
+ * clustering loops exits, and once on the main thread before the process
+ * exits; the next call on the thread re-creates them.  No counterpart in the
+ * reference (its driver keeps no per-thread native state). */
+int hz_release_thread(void);
 
 /* Opt-in HIP-event timing of the library's own launches (benchmark
  * instrumentation; no counterpart in the reference).  While enabled, each
@@ -78,8 +104,8 @@ int hz_profile_collect(double* ms_out, long long* launches_out, int n_kinds);
 
 /* Process-wide tuning knobs (0 = automatic choice), read when a workspace
  * layout is computed, so a workspace must be sized and used under the same
- * settings.  Used by the autotuner; no counterpart in the reference (its
- * closest analogue is the colsum_algo benchmark, _harmony/_helper.py:401-504). */
+ * settings.  No counterpart in the reference (its closest analogue is the
+ * colsum_algo selection, _harmony/_helper.py:401-504). */
 #define HZ_TUNE_ROW_CHUNK 0   /* rows per chunk of the row kernel (multiple of 16) */
 #define HZ_TUNE_CENTROID_WG 1 /* workgroups of the centroid pass */
 #define HZ_TUNE_SWEEP 2       /* 0 = block sweep in one launch when it fits, 1 = one launch per block */
@@ -87,6 +113,9 @@ int hz_profile_collect(double* ms_out, long long* launches_out, int n_kinds);
 #define HZ_TUNE_SWEEP_PLAIN 4 /* 1 = launch the block sweep as a plain dispatch of the same,
                                  co-resident size (for profilers that mishandle cooperative
                                  launches); 0 = cooperative launch */
+#define HZ_TUNE_SPIN_LIMIT 5 /* polls a block-sweep workgroup makes while waiting for the
+                                 others' block sums before it gives up and the call returns
+                                 HZ_ERR_LAUNCH (0 = 65536, about 65 ms) */
 #define HZ_TUNE_KEYS 8
 int hz_set_tuning(int key, int value);
 int hz_get_tuning(int key);
@@ -147,6 +176,47 @@ int hz_clustering_loop(const hz_dims* dims, const void* Z_norm, void* R, void* E
                        const void* Pr_b, const int32_t* cats, const void* theta, double sigma,
                        double tol, int max_iter, uint32_t seed, int stabilized, void* last_obj,
                        int* n_iter, void* ws, size_t ws_bytes, void* stream);
+
+/* One job of hz_clustering_loop_batched: the arguments of hz_clustering_loop
+ * for one Harmony instance (its own shape, state and workspace). */
+typedef struct hz_job {
+  hz_dims dims;
+  const void* Z_norm;
+  void* R;
+  void* E;
+  void* O;
+  const void* Pr_b;
+  const int32_t* cats;
+  const void* theta;
+  double sigma;
+  double tol;
+  int max_iter;
+  uint32_t seed;
+  int stabilized;
+  void* last_obj; /* device scalar */
+  int n_iter;     /* out: clustering iterations executed */
+  void* ws;
+  size_t ws_bytes;
+} hz_job;
+
+/* Clustering step of several independent Harmony instances at once: per
+ * clustering iteration, every job's centroid pass and block orders, then ONE
+ * cooperative launch in which each job's block sweep runs on its own share
+ * of the device's workgroups (shares proportional to the jobs' cell counts),
+ * so that the waits of one job's block exchange are filled by the other
+ * jobs' rows.  Each job's R, E, O, last_obj and n_iter are exactly those of
+ * hz_clustering_loop on that job alone (fixed-point sums: the result does not
+ * depend on the share).  All jobs must be float32 with one covariate, run
+ * the block sweep (block_size >= n_cells / 64, n_clusters <= 256), share the
+ * kernel shape (precision, ceil(n_clusters / 16) class and n_pcs class) and
+ * max_iter; otherwise HZ_ERR_UNSUPPORTED and nothing is launched.  bws is a
+ * device workspace of hz_batched_workspace_bytes bytes (per-iteration sweep
+ * arguments).  No counterpart in the reference (it runs one instance per
+ * call); the single-job interface it generalises is
+ * _cuda/harmony/clustering/clustering.cu:397-469. */
+size_t hz_batched_workspace_bytes(int n_jobs, int max_iter);
+int hz_clustering_loop_batched(hz_job* jobs, int n_jobs, void* bws, size_t bws_bytes,
+                               void* stream);
 
 /* Ridge correction from the ORIGINAL X: lambda_kb (written to lambda_kb_out,
  * n_batches x n_clusters), Z_hat and its L2-normalised rows Z_norm.

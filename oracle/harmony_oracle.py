@@ -501,3 +501,186 @@ def harmonize(X, cats, n_batches, *, Y0=None, n_clusters=None,
         record.update(objectives=objectives, clustering_iters=n_iters, R=R, E=E,
                       O=O, Z_norm=Z_norm)
     return Z_hat
+
+
+# --------------------------------------------------------------------------
+# Several covariates at once (stacked one-hot Phi).  Build extension: the
+# reference has no counterpart -- a list of keys is joined into one
+# categorical (_harmony/_helper.py:223-236).  Here each key is a covariate of
+# its own, as in the Harmony paper (Korsunsky et al. 2019): Phi stacks one
+# one-hot block per covariate, so every cell has one level in each.
+#
+# * O = Phi^T R (levels x K); E[b] = Pr_b (x) colsum(R), Pr_b = N_b / N per
+#   level (each covariate's Pr sums to 1).
+# * The R update of a cell multiplies the penalties of its levels:
+#   R_ik ~ exp(-2/sigma (1 - sim_ik)) * prod_b pen_bk^(Phi_ib), which is the
+#   minimiser of the objective below for fixed O and E (for one covariate it
+#   is the reference's update).
+# * Objective: k-means + sigma * entropy + sigma * sum over all levels of
+#   theta_b O_bk log(numer / (E + 1)).
+# * Correction per cluster k: A_k = Phi~^T diag(R_k) Phi~ + diag(0, lambda_k)
+#   with Phi~ = [1 | Phi]; W_k = A_k^{-1} Phi~^T diag(R_k) X; W_k[0] = 0;
+#   Z_i = X_i - sum_k R_ik sum_{b: Phi_ib = 1} W_k[1 + b].  A_k is no longer
+#   an arrowhead matrix (the cross-covariate blocks hold co-occurrence sums),
+#   so it is solved densely.
+# --------------------------------------------------------------------------
+
+
+def multi_levels(codes_list, n_levels_list):
+    """Global level ids (N x C, covariate c's codes offset by the levels of the
+    covariates before it) and the per-level cell counts."""
+    offs = np.concatenate([[0], np.cumsum(n_levels_list)[:-1]]).astype(np.int64)
+    lev = np.stack([np.asarray(c, np.int64) + o for c, o in zip(codes_list, offs)], axis=1)
+    n_b = np.bincount(lev.ravel(), minlength=int(np.sum(n_levels_list)))
+    return lev.astype(np.int32), n_b
+
+
+def multi_theta(theta, n_levels_list, dtype, *, tau=0, n_b=None, n_clusters=None):
+    """theta per level: a scalar, or one value per covariate broadcast to its
+    levels, or one value per level; then the reference's tau discount
+    (_harmony/__init__.py:172-175) per level."""
+    dt = np.dtype(dtype)
+    total = int(np.sum(n_levels_list))
+    if isinstance(theta, (int, float)):
+        th = np.full(total, float(theta), dtype=dt)
+    else:
+        th = np.asarray(theta, dtype=dt).ravel()
+        if th.size == len(n_levels_list):
+            th = np.repeat(th, n_levels_list).astype(dt)
+        elif th.size != total:
+            raise ValueError(f"theta needs 1, {len(n_levels_list)} or {total} values")
+    if tau > 0:
+        th = th * (1 - np.exp(-np.asarray(n_b, dt) / (n_clusters * tau)) ** 2)
+    return np.ascontiguousarray(th.astype(dt))
+
+
+def _phi_rows(lev, rows, n_levels):
+    """Phi^T rows: the sum of ``rows`` over the cells of every level."""
+    out = np.zeros((n_levels, rows.shape[1]), dtype=rows.dtype)
+    for c in range(lev.shape[1]):
+        out += _scatter_rows(lev[:, c], rows, n_levels)
+    return out
+
+
+def multi_penalty_rows(pen, lev):
+    """Per-cell penalty: the product of the cell's level penalties."""
+    out = pen[lev[:, 0]].copy()
+    for c in range(1, lev.shape[1]):
+        out *= pen[lev[:, c]]
+    return out
+
+
+def multi_init(Z_norm, Y0, *, sigma, Pr_b, theta, lev, n_levels, stabilized):
+    dt = Z_norm.dtype
+    Y_norm = l2_normalize(np.asarray(Y0, dtype=dt))
+    sim = Z_norm @ Y_norm.T
+    R = l1_normalize(np.exp(dt.type(-2 / sigma) * (dt.type(1) - sim)))
+    E = (Pr_b.reshape(-1, 1) * R.sum(axis=0)[None, :]).astype(dt)
+    O = _phi_rows(lev, R, n_levels)
+    obj = objective(R, sim, O, E, theta, sigma, stabilized)
+    return R, E, O, float(obj)
+
+
+def multi_clustering_loop(Z_norm, R, E, O, Pr_b, lev, theta, *, sigma, tol, max_iter,
+                          block_size, seed, stabilized):
+    """clustering_loop with stacked one-hot Phi (same block order, same
+    remove -> penalty -> rows -> add order, same window-3 test)."""
+    dt = R.dtype.type
+    n = R.shape[0]
+    L = O.shape[0]
+    term = dt(-2) / dt(sigma)
+    tol_t = dt(tol)
+    Pr = Pr_b.reshape(-1)
+    objectives = []
+    for it in range(max_iter):
+        Y_norm = l2_normalize(R.T @ Z_norm)
+        sim = Z_norm @ Y_norm.T
+        order = block_order(n, (seed + it) & _U32)
+        for pos in range(0, n, block_size):
+            idx = order[pos:pos + block_size]
+            R_in = R[idx]
+            lv = lev[idx]
+            O -= _phi_rows(lv, R_in, L)
+            E -= Pr[:, None] * R_in.sum(axis=0)[None, :]
+            pen = penalty(E, O, theta, stabilized)
+            v = np.exp(term * (dt(1) - sim[idx])) * multi_penalty_rows(pen, lv)
+            R_out = v * (dt(1) / v.sum(axis=1))[:, None]
+            R[idx] = R_out
+            O += _phi_rows(lv, R_out, L)
+            E += Pr[:, None] * R_out.sum(axis=0)[None, :]
+        objectives.append(objective(R, sim, O, E, theta, sigma, stabilized))
+        if len(objectives) >= 4:
+            old = dt(0)
+            new = dt(0)
+            for w in range(3):
+                old = dt(old + objectives[-2 - w])
+                new = dt(new + objectives[-1 - w])
+            if (old - new) < tol_t * abs(old):
+                break
+    last = objectives[-1] if objectives else dt(0)
+    return float(last), len(objectives), [float(o) for o in objectives]
+
+
+def multi_correction(X, R, lambda_kb, lev, n_levels):
+    """Ridge correction with stacked one-hot Phi (dense solve per cluster).
+    Returns (Z_hat, W) with W of shape K x (levels + 1) x d."""
+    dt = X.dtype
+    n, d = X.shape
+    K = R.shape[1]
+    P = np.zeros((n, n_levels + 1), dtype=dt)
+    P[:, 0] = 1
+    for c in range(lev.shape[1]):
+        P[np.arange(n), 1 + lev[:, c]] = 1
+    W = np.zeros((K, n_levels + 1, d), dtype=dt)
+    Z = X.copy()
+    for k in range(K):
+        PR = P * R[:, k:k + 1]
+        A = PR.T @ P
+        A[np.arange(1, n_levels + 1), np.arange(1, n_levels + 1)] += lambda_kb[:, k]
+        W[k] = np.linalg.solve(A, PR.T @ X)
+        W[k, 0] = 0
+        Z -= R[:, k:k + 1] * (P @ W[k])
+    return Z.astype(dt), W
+
+
+def harmonize_multi(X, codes_list, n_levels_list, *, Y0, n_clusters=None, max_iter_harmony=10,
+                    max_iter_clustering=200, tol_harmony=1e-4, tol_clustering=1e-5,
+                    ridge_lambda=1.0, sigma=0.1, block_proportion=0.05, theta=2.0, tau=0,
+                    random_state=0, stabilized_penalty=True, dynamic_lambda=True, alpha=0.2,
+                    batch_prune_threshold=1e-5, record=None):
+    """Whole Harmony run with one covariate per entry of ``codes_list``
+    (stacked one-hot Phi); same driver as ``harmonize`` otherwise."""
+    X = np.ascontiguousarray(X)
+    dt = X.dtype
+    n = X.shape[0]
+    lev, n_b = multi_levels(codes_list, n_levels_list)
+    L = int(np.sum(n_levels_list))
+    N_b = n_b.astype(dt)
+    Pr_b = (N_b.reshape(-1, 1) / n).astype(dt)
+    if n_clusters is None:
+        n_clusters = max(int(min(100, n / 30)), 2)
+    th = multi_theta(theta, n_levels_list, dt, tau=tau, n_b=N_b, n_clusters=n_clusters)
+    Z_norm = l2_normalize(X)
+    R, E, O, obj0 = multi_init(Z_norm, Y0, sigma=sigma, Pr_b=Pr_b, theta=th, lev=lev,
+                               n_levels=L, stabilized=stabilized_penalty)
+    objectives = [obj0]
+    block_size = int(n * block_proportion)
+    Z_hat = X
+    n_iters = []
+    for i in range(max_iter_harmony):
+        last, nc, _ = multi_clustering_loop(
+            Z_norm, R, E, O, Pr_b, lev, th, sigma=sigma, tol=tol_clustering,
+            max_iter=max_iter_clustering, block_size=block_size,
+            seed=(random_state + i * 1000003) & _U32, stabilized=stabilized_penalty)
+        objectives.append(last)
+        n_iters.append(nc)
+        lam = compute_lambda_kb(E, O, N_b, alpha=alpha, threshold=batch_prune_threshold,
+                                ridge_lambda=ridge_lambda, dynamic_lambda=dynamic_lambda)
+        Z_hat, _ = multi_correction(X, R, lam, lev, L)
+        Z_norm = l2_normalize(Z_hat)
+        if (objectives[-2] - objectives[-1]) < tol_harmony * abs(objectives[-2]):
+            break
+    if record is not None:
+        record.update(objectives=objectives, clustering_iters=n_iters, R=R, E=E, O=O,
+                      Z_norm=Z_norm, levels=lev)
+    return Z_hat

@@ -7,6 +7,7 @@ Plays the role of the reference's lazy nanobind module loader
 
 from __future__ import annotations
 
+import atexit
 import ctypes
 import os
 from pathlib import Path
@@ -19,7 +20,8 @@ HZ_F64 = 8
 HZ_PREC_NATIVE = 0
 HZ_PREC_SPLIT_BF16 = 1
 HZ_CORR = {"batched": 0, "fast": 1, "original": 2}
-TUNE_KEYS = {"row_chunk": 0, "centroid_wg": 1, "no_sweep": 2, "sweep_wg": 3, "sweep_plain": 4}
+TUNE_KEYS = {"row_chunk": 0, "centroid_wg": 1, "no_sweep": 2, "sweep_wg": 3, "sweep_plain": 4,
+             "spin_limit": 5}
 PROF_KINDS = ["row", "centroid", "yreduce", "order", "correction", "phi", "apply", "sweep"]
 
 _vp = ctypes.c_void_p
@@ -39,6 +41,10 @@ class HzDims(ctypes.Structure):
         ("block_size", _i),
         ("dtype", _i),
         ("precision", _i),
+        ("n_covariates", _i),
+        ("n_levels", _i),
+        ("n_levels0", _i),
+        ("group_levels", _vp),
     ]
 
 
@@ -47,6 +53,7 @@ _P = ctypes.POINTER(HzDims)
 # name -> (restype, argtypes)
 _SIGNATURES = {
     "hz_version": (_i, []),
+    "hz_release_thread": (_i, []),
     "hz_profile_enable": (_i, [_i]),
     "hz_set_tuning": (_i, [_i, _i]),
     "hz_get_tuning": (_i, [_i]),
@@ -67,6 +74,8 @@ _SIGNATURES = {
         [_P, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _i, _u32, _i, _vp,
          ctypes.POINTER(_i), _vp, _sz, _vp],
     ),
+    "hz_batched_workspace_bytes": (_sz, [_i, _i]),
+    "hz_clustering_loop_batched": (_i, [_vp, _i, _vp, _sz, _vp]),
     "hz_correction": (
         _i,
         [_P, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _d, _i, _d, _d, _vp, _vp, _vp,
@@ -108,7 +117,17 @@ def load() -> ctypes.CDLL:
         fn.restype = res
         fn.argtypes = args
     _lib = lib
+    # the main thread's side stream, events and pinned flags are released
+    # before the interpreter (and the HIP runtime under it) shuts down
+    atexit.register(release_thread)
     return lib
+
+
+def release_thread() -> None:
+    """Release the library's per-thread device resources of the calling
+    thread (side stream, events, pinned flags); recreated on next use."""
+    if _lib is not None:
+        _lib.hz_release_thread()
 
 
 def exported_symbols() -> list[str]:
@@ -126,9 +145,13 @@ def check(rc: int, what: str) -> None:
         raise HarmonyNativeError(f"{what} failed (status {rc}): {msg}")
 
 
-def dims(n_cells, n_pcs, n_clusters, n_batches, block_size, dtype_size, precision=0):
+def dims(n_cells, n_pcs, n_clusters, n_batches, block_size, dtype_size, precision=0,
+         n_covariates=1, n_levels=0, n_levels0=0, group_levels=None):
+    """``hz_dims``; for several covariates ``n_batches`` counts the groups of
+    cells sharing all levels and ``group_levels`` is a device pointer."""
     return HzDims(int(n_cells), int(n_pcs), int(n_clusters), int(n_batches),
-                  int(block_size), int(dtype_size), int(precision))
+                  int(block_size), int(dtype_size), int(precision), int(n_covariates),
+                  int(n_levels), int(n_levels0), group_levels)
 
 
 def profile_enable(on: bool) -> None:

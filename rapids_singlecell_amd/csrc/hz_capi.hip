@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -183,6 +184,20 @@ static int make_layout(const hz_dims* dm, void* base, Layout& L) {
   L.d = dm->n_pcs;
   L.K = dm->n_clusters;
   L.B = dm->n_batches;
+  L.C = std::max(1, dm->n_covariates);
+  if (L.C >= 2) {
+    HZ_RET_IF(dm->n_levels < L.C || dm->n_levels0 < 1 || dm->n_levels0 >= dm->n_levels,
+              HZ_ERR_ARG, "several covariates need n_levels >= n_covariates and "
+              "1 <= n_levels0 < n_levels (got %d, %d)", dm->n_levels, dm->n_levels0);
+    HZ_RET_IF(!dm->group_levels, HZ_ERR_ARG, "several covariates need group_levels");
+    L.BL = dm->n_levels;
+    L.BL0 = dm->n_levels0;
+    L.glev = dm->group_levels;
+  } else {
+    L.BL = L.B;
+    L.BL0 = L.B;
+    L.glev = nullptr;
+  }
   L.bs = std::min(dm->block_size, dm->n_cells);
   L.esz = dm->dtype;
   L.CT = pick_ct(L.K);
@@ -190,8 +205,8 @@ static int make_layout(const hz_dims* dm, void* base, Layout& L) {
   HZ_RET_IF(!L.CT, HZ_ERR_UNSUPPORTED, "n_clusters=%d > 256 is not supported", L.K);
   HZ_RET_IF(!L.DT, HZ_ERR_UNSUPPORTED, "n_pcs=%d > 127 is not supported", L.d);
   {
-    const int need = (L.esz == 4) ? block_update_min_lds<float>(L.B) : block_update_min_lds<double>(L.B);
-    HZ_RET_IF(need > kYRedLds, HZ_ERR_UNSUPPORTED, "n_batches=%d is too many (limit %d)", L.B,
+    const int need = (L.esz == 4) ? block_update_min_lds<float>(L.BL) : block_update_min_lds<double>(L.BL);
+    HZ_RET_IF(need > kYRedLds, HZ_ERR_UNSUPPORTED, "%d batch levels are too many (limit %d)", L.BL,
               (kYRedLds - 16) / (16 + 2 * L.esz));
   }
   L.Kp = 16 * L.CT;
@@ -254,8 +269,11 @@ static int make_layout(const hz_dims* dm, void* base, Layout& L) {
     L.G = std::max(L.G, L.Gs);
   }
   const long long nseg = (long long)L.pd.nblk * L.B;
+  L.nacc = (long long)L.pd.nblk * L.BL;
   const long long BK = (long long)L.B * L.K;
+  const long long BLK = (long long)L.BL * L.K;
   const long long KD = (long long)L.K * L.d;
+  (void)nseg;
 
   Carve c(base);
   carve_perm(c, L.pd, L.pb, L.ct, &L.nch_tmp, L.nord, !member_fits(L.pd));
@@ -277,8 +295,8 @@ static int make_layout(const hz_dims* dm, void* base, Layout& L) {
     L.sp.ch_seg = c.take<int>(L.spd.maxch);
     L.nch_tmp_p = c.take<int>(L.B + 1);
   }
-  L.acc_in = c.take<long long>(nseg * L.K);
-  L.acc_out = c.take<long long>((long long)kRep * nseg * L.K);
+  L.acc_in = c.take<long long>(L.nacc * L.K);
+  L.acc_out = c.take<long long>((long long)kRep * L.nacc * L.K);
   L.obj_acc = c.take<long long>(2 * kRep);
   L.ctrl = c.take<Ctrl>(1);
   L.ctrl_static = c.take<Ctrl>(1);
@@ -286,20 +304,42 @@ static int make_layout(const hz_dims* dm, void* base, Layout& L) {
   L.swcnt = c.take<int>((long long)L.pd.nblk * L.B * kSweepRep);
   L.swtrash = c.take<float>(256);
   L.Ypart = c.take<char>((size_t)L.G * KD * L.esz);
+  L.Yslice = c.take<double>((size_t)kYSlices * KD);
+  L.ycnt = c.take<int>(L.K);
   L.Y = c.take<char>((size_t)KD * L.esz);
   L.YT = c.take<char>((size_t)L.d * L.Kp * L.esz);
   L.YH = c.take<char>((size_t)L.Kp * L.dP * 2);
   L.YL = c.take<char>((size_t)L.Kp * L.dP * 2);
-  L.pen = c.take<char>((size_t)BK * L.esz);
+  L.pen = c.take<char>((size_t)BLK * L.esz);
   L.phipart = c.take<char>((size_t)L.spd.maxch * KD * L.esz);
   L.Phi = c.take<double>(BK * L.d);
   L.rsum = c.take<long long>(BK);
-  L.lam = c.take<char>((size_t)BK * L.esz);
+  L.lam = c.take<char>((size_t)BLK * L.esz);
+  // several covariates: W of every level, K x (BL + 1) x d, FP64
+  L.Wl = (L.C >= 2) ? c.take<double>((size_t)KD * (L.BL + 1)) : nullptr;
   L.Wt = c.take<char>((size_t)BK * L.d * L.esz);
   L.bytes = align_up(c.off, 256);
   L.scale_o = std::ldexp(1.0, fixed_shift((unsigned long long)L.n + 1));
   L.scale_obj = std::ldexp(1.0, fixed_shift(16ull * (unsigned long long)L.n + 16));
   return HZ_OK;
+}
+
+static Levels levels_of(const Layout& L) {
+  Levels lv;
+  lv.glev = L.glev;
+  lv.C = L.C;
+  lv.Bg = L.B;
+  lv.BL = L.BL;
+  return lv;
+}
+
+// the same segments counted as their own levels (group sums)
+static Levels group_levels_of(const Layout& L) {
+  Levels lv = levels_of(L);
+  lv.glev = nullptr;
+  lv.C = 1;
+  lv.BL = L.B;
+  return lv;
 }
 
 template <typename T>
@@ -314,10 +354,12 @@ static EpiArgs<T> make_epi(const Layout& L, T* O, T* E, const T* Pr, const T* th
   e.acc_in = L.acc_in;
   e.acc_out = L.acc_out;
   e.obj_acc = L.obj_acc;
-  e.rep_stride = (long long)L.pd.nblk * L.B * L.K;
+  e.rep_stride = L.nacc * L.K;
   e.ctrl = L.ctrl;
   e.last_obj = last_obj;
-  e.B = L.B;
+  e.B = L.BL;
+  e.Bcol = L.BL0;
+  e.lv = levels_of(L);
   e.K = L.K;
   e.nblk = L.pd.nblk;
   e.sigma = T(sigma);
@@ -341,7 +383,7 @@ template <typename T>
 static size_t row_lds_bytes(const Layout& L) {
   size_t need = (size_t)L.d * L.Kp * sizeof(T) + (size_t)L.d * kZtStride * sizeof(T) +
                 (size_t)(kBlock / kWave) * L.Kp * sizeof(double);
-  return std::max(need, (size_t)block_update_min_lds<T>(L.B));
+  return std::max(need, (size_t)block_update_min_lds<T>(L.BL));
 }
 
 template <typename T, int CT>
@@ -468,6 +510,14 @@ static int checks_push(DoneChecks& c, const Ctrl* ctrl, hipStream_t s) {
   return HZ_OK;
 }
 
+static int checks_push_flag(DoneChecks& c, const int* flag, hipStream_t s) {
+  const int slot = (c.head + c.count) % DoneChecks::kSlots;
+  HZ_HIP(hipMemcpyAsync(&c.host[slot], flag, sizeof(int), hipMemcpyDeviceToHost, s));
+  HZ_HIP(hipEventRecord(c.ev[slot], s));
+  ++c.count;
+  return HZ_OK;
+}
+
 // Pops completed checks (blocking on the oldest while more than `max_pending`
 // remain); *done = 1 once a copied flag is set.
 static int checks_poll(DoneChecks& c, int max_pending, int* done) {
@@ -516,6 +566,76 @@ static int side_stream(SideStream** out) {
   return HZ_OK;
 }
 
+// Releases the calling thread's side streams, their events and the pinned
+// done-flag rings on every device it used (after waiting for their work).
+static int release_thread_resources() {
+  int rc = HZ_OK;
+  int cur = 0;
+  const bool have_cur = hipGetDevice(&cur) == hipSuccess;
+  for (int dev = 0; dev < 64; ++dev) {
+    SideStream& ss = t_side[dev];
+    DoneChecks& c = t_checks[dev];
+    if (!ss.s && !c.host) continue;
+    if (hipSetDevice(dev) != hipSuccess) {
+      rc = HZ_ERR_HIP;
+      continue;
+    }
+    if (ss.s) {
+      if (hipStreamSynchronize(ss.s) != hipSuccess) rc = HZ_ERR_HIP;
+      hipEvent_t* evs[] = {&ss.start, &ss.done, &ss.ready[0], &ss.ready[1],
+                           &ss.freed[0], &ss.freed[1], &ss.tabs};
+      for (hipEvent_t* e : evs)
+        if (*e) {
+          (void)hipEventDestroy(*e);
+          *e = nullptr;
+        }
+      if (hipStreamDestroy(ss.s) != hipSuccess) rc = HZ_ERR_HIP;
+      ss.s = nullptr;
+    }
+    if (c.host) {
+      for (auto& e : c.ev)
+        if (e) {
+          (void)hipEventSynchronize(e);
+          (void)hipEventDestroy(e);
+          e = nullptr;
+        }
+      if (hipHostFree(c.host) != hipSuccess) rc = HZ_ERR_HIP;
+      c.host = nullptr;
+      c.head = c.count = 0;
+    }
+  }
+  if (have_cur) (void)hipSetDevice(cur);
+  if (rc != HZ_OK) set_error("releasing the thread's streams, events or pinned flags failed");
+  return rc;
+}
+
+// Device-wide order of the block sweeps.  A sweep is a cooperative launch
+// whose workgroups wait for each other, so two sweeps (or two multi-job
+// sweeps) must never hold the device's CUs at the same time: a second one
+// that got only part of the CUs would wait for workgroups that cannot start.
+// Each sweep launch therefore waits, on its own stream, for the previous
+// sweep launched on the same device from any thread or stream.
+struct SweepChain {
+  std::mutex mu;
+  hipEvent_t last = nullptr;
+  bool recorded = false;
+};
+static SweepChain g_chain[64];
+
+template <typename F>
+static int chained_sweep(hipStream_t s, F&& launch) {
+  SweepChain& ch = g_chain[current_device_index()];
+  std::lock_guard<std::mutex> lock(ch.mu);
+  if (!ch.last) HZ_HIP(hipEventCreateWithFlags(&ch.last, hipEventDisableTiming));
+  if (ch.recorded) HZ_HIP(hipStreamWaitEvent(s, ch.last, 0));
+  const int rc = launch();
+  if (rc == HZ_OK) {
+    HZ_HIP(hipEventRecord(ch.last, s));
+    ch.recorded = true;
+  }
+  return rc;
+}
+
 // ---------------------------------------------------------------- jobs
 
 template <typename T>
@@ -531,7 +651,7 @@ static int init_impl(const hz_dims* dm, const T* Z, const T* Y0, const int* cats
   HZ_TRY(launch_static_chunks(L.n, L.B, L.sd.chunk, cat_offsets, cell_indices, L.st, L.nch_tmp_s, s));
   hipLaunchKernelGGL(ctrl_reset_kernel, dim3(1), dim3(64), 0, s, L.ctrl);
   HZ_LAUNCHED("ctrl_reset");
-  HZ_TRY(fill<long long>(L.acc_out, (long long)kRep * L.pd.nblk * L.B * L.K, 0, s));
+  HZ_TRY(fill<long long>(L.acc_out, (long long)kRep * L.nacc * L.K, 0, s));
   HZ_TRY(fill<long long>(L.obj_acc, 2 * kRep, 0, s));
   HZ_TRY(fill<int>(L.counters, L.pd.nblk + 6, 0, s));
   hipLaunchKernelGGL(centroids_to_yt_kernel<T>, dim3(1), dim3(kBlock), 0, s, Y0,
@@ -584,6 +704,66 @@ static int init_impl(const hz_dims* dm, const T* Z, const T* Y0, const int* cats
 }
 
 
+// Block-sweep arguments of one job (tables of the first iteration; the
+// caller binds each iteration's).
+static SweepArgs make_sweep_args(const Layout& L, const float* Z, float* R, float* O, float* E,
+                                 const float* Pr, const float* theta, double sigma, double tol,
+                                 int stabilized, float* last_obj) {
+  SweepArgs sw{};
+  sw.Z = Z;
+  sw.YT = reinterpret_cast<const float*>(L.YT);
+  sw.YH = reinterpret_cast<const __bf16*>(L.YH);
+  sw.YL = reinterpret_cast<const __bf16*>(L.YL);
+  sw.R = R;
+  sw.order = L.ct.order;
+  sw.seg_ch = L.ct.seg_ch;
+  sw.ch_start = L.ct.ch_start;
+  sw.ch_len = L.ct.ch_len;
+  sw.ch_seg = L.ct.ch_seg;
+  sw.acc_in = L.acc_in;
+  sw.acc_out = L.acc_out;
+  sw.obj_acc = L.obj_acc;
+  sw.rep_stride = L.nacc * L.K;
+  sw.O = O;
+  sw.E = E;
+  sw.Pr = Pr;
+  sw.theta = theta;
+  sw.ctrl = L.ctrl;
+  sw.last_obj = last_obj;
+  sw.fin = L.counters + L.pd.nblk + 3;
+  sw.n = L.n;
+  sw.d = L.d;
+  sw.K = L.K;
+  sw.Kp = L.Kp;
+  sw.dP = L.dP;
+  sw.B = L.B;
+  sw.nblk = L.pd.nblk;
+  sw.term = (float)(-2.0 / sigma);
+  sw.sigma = (float)sigma;
+  sw.tol = (float)tol;
+  sw.stabilized = stabilized;
+  sw.scale_o = L.scale_o;
+  sw.inv_scale_o = 1.0 / L.scale_o;
+  sw.scale_obj = L.scale_obj;
+  sw.inv_scale_obj = 1.0 / L.scale_obj;
+  sw.cnt = L.swcnt;
+  sw.trash = L.swtrash;
+  // counted block sums: value field below bit 53 holds at most N * 2^s
+  sw.scale_pk = std::ldexp(1.0, std::min(40, 52 - bit_length((unsigned long long)L.n + 1)));
+  sw.inv_scale_pk = 1.0 / sw.scale_pk;
+  // polls of about a microsecond: a wait that long is a fault
+  sw.spin_limit = g_tune[HZ_TUNE_SPIN_LIMIT] > 0 ? (unsigned)g_tune[HZ_TUNE_SPIN_LIMIT] : 1u << 16;
+  return sw;
+}
+
+static void bind_sweep_tab(SweepArgs& sw, const ChunkTab& t) {
+  sw.order = t.order;
+  sw.seg_ch = t.seg_ch;
+  sw.ch_start = t.ch_start;
+  sw.ch_len = t.ch_len;
+  sw.ch_seg = t.ch_seg;
+}
+
 template <typename T>
 static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, const T* Pr,
                            const int* cats, const T* theta, double sigma, double tol, int max_iter,
@@ -603,6 +783,7 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
   HZ_LAUNCHED("ctrl_reset");
   // acc_out / obj_acc are cleared by the centroid reduction of every iteration
   HZ_TRY(fill<int>(L.counters, L.pd.nblk + 6, 0, s));
+  HZ_TRY(fill<int>(L.ycnt, L.K, 0, s));
   if (last_obj) HZ_TRY(fill<T>(last_obj, 1, T(0), s));
 
   const EpiArgs<T> epi = make_epi<T>(L, O, E, Pr, theta, sigma, tol, stabilized, last_obj);
@@ -623,6 +804,7 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
   ca.d = L.d;
   ca.K = L.K;
   ca.scale_o = L.scale_o;
+  ca.lv = levels_of(L);
 
   YRedArgs<T> ya;
   ya.Ypart = reinterpret_cast<const T*>(L.Ypart);
@@ -634,7 +816,9 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
   ya.Kp = L.Kp;
   ya.setup_block0 = 1;
   ya.epi = epi;
-  const int ygrid = L.K;
+  ya.Yslice = L.Yslice;
+  ya.ycnt = L.ycnt;
+  const dim3 ygrid(L.K, kYSlices);
 
   RowArgs<T> ra;
   ra.Z = Z;
@@ -691,6 +875,7 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
     cm.d = L.d;
     cm.K = L.K;
     cm.scale_o = L.scale_o;
+    cm.lv = levels_of(L);
   }
   bool use_sw = false;
   int sweep_grid = 1;  // workgroups of the block sweep (all co-resident)
@@ -702,51 +887,11 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
     const bool aligned = (reinterpret_cast<uintptr_t>(Z) % 16 == 0) &&
                          (reinterpret_cast<uintptr_t>(R) % 16 == 0);
     // one 16-row MFMA tile per chunk
+    // the sweep keeps one covariate's O/E/penalty per workgroup in LDS
     use_sw = use_rm && aligned && L.chunk == 16 && L.K <= 256 && L.pd.nblk <= kSweepMaxBlocks &&
-             g_tune[HZ_TUNE_SWEEP] == 0 &&
+             L.C == 1 && g_tune[HZ_TUNE_SWEEP] == 0 &&
              !sweep_broken();
-    sw.Z = Z;
-    sw.YT = reinterpret_cast<const float*>(L.YT);
-    sw.YH = reinterpret_cast<const __bf16*>(L.YH);
-    sw.YL = reinterpret_cast<const __bf16*>(L.YL);
-    sw.R = R;
-    sw.order = L.ct.order;
-    sw.seg_ch = L.ct.seg_ch;
-    sw.ch_start = L.ct.ch_start;
-    sw.ch_len = L.ct.ch_len;
-    sw.ch_seg = L.ct.ch_seg;
-    sw.acc_in = L.acc_in;
-    sw.acc_out = L.acc_out;
-    sw.obj_acc = L.obj_acc;
-    sw.rep_stride = epi.rep_stride;
-    sw.O = O;
-    sw.E = E;
-    sw.Pr = Pr;
-    sw.theta = theta;
-    sw.ctrl = L.ctrl;
-    sw.last_obj = last_obj;
-    sw.fin = L.counters + L.pd.nblk + 3;
-    sw.n = L.n;
-    sw.d = L.d;
-    sw.K = L.K;
-    sw.Kp = L.Kp;
-    sw.dP = L.dP;
-    sw.B = L.B;
-    sw.nblk = L.pd.nblk;
-    sw.term = ra.term;
-    sw.sigma = (float)sigma;
-    sw.tol = (float)tol;
-    sw.stabilized = stabilized;
-    sw.scale_o = L.scale_o;
-    sw.inv_scale_o = 1.0 / L.scale_o;
-    sw.scale_obj = L.scale_obj;
-    sw.inv_scale_obj = 1.0 / L.scale_obj;
-    sw.cnt = L.swcnt;
-    sw.trash = L.swtrash;
-    // counted block sums: value field below bit 53 holds at most N * 2^s
-    sw.scale_pk = std::ldexp(1.0, std::min(40, 52 - bit_length((unsigned long long)L.n + 1)));
-    sw.inv_scale_pk = 1.0 / sw.scale_pk;
-    sw.spin_limit = 1u << 16;  // polls of about a microsecond: a wait that long is a fault
+    sw = make_sweep_args(L, Z, R, O, E, Pr, theta, sigma, tol, stabilized, last_obj);
     cs.Z = Z;
     cs.R = R;
     cs.Ypart = reinterpret_cast<float*>(L.Ypart);
@@ -837,7 +982,7 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
       ia.ch_len = t.ch_len;
       ia.ch_seg = t.ch_seg;
       ProfScope ps(HZ_PROF_ORDER, s2);
-      HZ_TRY(fill<long long>(L.acc_in, nseg * L.K, 0, s2));
+      HZ_TRY(fill<long long>(L.acc_in, L.nacc * L.K, 0, s2));
       HZ_TRY(launch_insum(L, ia, s2));
       hipLaunchKernelGGL(sweep_count_kernel, dim3(L.pd.nblk), dim3(kBlock),
                          (size_t)L.B * kSweepRep * sizeof(int), s2, &L.ctrl->done, t.seg_ch,
@@ -857,7 +1002,7 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
         ProfScope ps(HZ_PROF_YREDUCE, s);
         ya.G = gs;
         ya.setup_block0 = 0;
-        hipLaunchKernelGGL(yreduce_kernel<T>, dim3(ygrid), dim3(kBlock), 0, s, ya);
+        hipLaunchKernelGGL(yreduce_kernel<T>, ygrid, dim3(kBlock), 0, s, ya);
         HZ_LAUNCHED("yreduce_kernel");
         HZ_TRY(launch_split(L, L.ctrl, s));
       }
@@ -865,7 +1010,7 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
       int rc;
       {
         ProfScope ps(HZ_PROF_SWEEP, s);
-        rc = launch_sweep(L, &sw, &sweep_grid, s);
+        rc = chained_sweep(s, [&]() { return launch_sweep(L, &sw, &sweep_grid, s); });
       }
       if (rc == HZ_OK) {
         iter_done = true;
@@ -877,7 +1022,7 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
     }
     if (!iter_done) {
       HZ_HIP(hipStreamWaitEvent(s, ss->ready[it & 1], 0));
-      HZ_TRY(fill<long long>(L.acc_in, nseg * L.K, 0, s));
+      HZ_TRY(fill<long long>(L.acc_in, L.nacc * L.K, 0, s));
       {
         ProfScope ps(HZ_PROF_CENTROID, s);
         if (use_cm)
@@ -889,7 +1034,7 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
         ProfScope ps(HZ_PROF_YREDUCE, s);
         ya.G = use_cm ? L.Gm : L.G;
         ya.setup_block0 = 1;
-        hipLaunchKernelGGL(yreduce_kernel<T>, dim3(ygrid), dim3(kBlock), 0, s, ya);
+        hipLaunchKernelGGL(yreduce_kernel<T>, ygrid, dim3(kBlock), 0, s, ya);
         HZ_LAUNCHED("yreduce_kernel");
         if (use_rm) HZ_TRY(launch_split(L, L.ctrl, s));
       }
@@ -925,6 +1070,263 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
   return HZ_OK;
 }
 
+// ---------------------------------------------------------------- several jobs
+
+// AND of the jobs' convergence flags (one thread).
+static __global__ void all_done_kernel(Ctrl* const* __restrict__ ctrls, int n, int* out) {
+  if (threadIdx.x == 0) {
+    int all = 1;
+    for (int j = 0; j < n; ++j) all &= (ctrls[j]->done | ctrls[j]->pad0) ? 1 : 0;
+    *out = all;
+  }
+}
+
+// Device workspace of the batched loop: per-(iteration, job) sweep arguments,
+// workgroup offsets, control-block pointers and the all-done flag.
+static size_t batched_bytes(int n_jobs, int max_iter) {
+  Carve c(nullptr);
+  (void)c.take<SweepArgs>((size_t)std::max(1, max_iter) * n_jobs);
+  (void)c.take<int>(n_jobs + 1);
+  (void)c.take<Ctrl*>(n_jobs);
+  (void)c.take<int>(64);
+  return align_up(c.off, 256);
+}
+
+struct BatchJob {
+  Layout L;
+  EpiArgs<float> epi;
+  CentSArgs cs;
+  YRedArgs<float> ya;
+  InSumArgs ia;
+  SweepArgs sw;
+  int G = 1;
+};
+
+static int batched_impl(hz_job* jobs, int n, void* bws, size_t bws_bytes, hipStream_t s) {
+  HZ_RET_IF(n < 1 || !jobs, HZ_ERR_ARG, "no jobs");
+  const int max_iter = jobs[0].max_iter;
+  for (int j = 0; j < n; ++j) {
+    jobs[j].n_iter = 0;
+    HZ_RET_IF(jobs[j].max_iter != max_iter, HZ_ERR_ARG,
+              "every job of a batch needs the same max_iter (job %d: %d, job 0: %d)", j,
+              jobs[j].max_iter, max_iter);
+  }
+  std::vector<BatchJob> J(n);
+  long long cells = 0;
+  for (int j = 0; j < n; ++j) {
+    hz_job& h = jobs[j];
+    Layout& L = J[j].L;
+    HZ_TRY(make_layout(&h.dims, nullptr, L));
+    HZ_TRY(check_ws(L, h.ws, h.ws_bytes));
+    HZ_TRY(make_layout(&h.dims, h.ws, L));
+    const bool aligned = (reinterpret_cast<uintptr_t>(h.Z_norm) % 16 == 0) &&
+                         (reinterpret_cast<uintptr_t>(h.R) % 16 == 0);
+    HZ_RET_IF(L.esz != 4 || !L.mfma || L.chunk != 16 || L.K > 256 || L.C != 1 || !aligned ||
+                  L.pd.nblk > kSweepMaxBlocks || g_tune[HZ_TUNE_SWEEP] != 0 || sweep_broken(),
+              HZ_ERR_UNSUPPORTED, "job %d cannot run in the batched block sweep", j);
+    const Layout& L0 = J[0].L;
+    HZ_RET_IF(L.prec != L0.prec || L.NT != L0.NT || row_sm(L) != row_sm(L0), HZ_ERR_UNSUPPORTED,
+              "job %d has another kernel shape than job 0 (precision, cluster or PC class)", j);
+    cells += L.n;
+  }
+  const size_t need = batched_bytes(n, max_iter);
+  HZ_RET_IF(!bws || bws_bytes < need, HZ_ERR_WORKSPACE, "batched workspace too small: %zu < %zu",
+            bws_bytes, need);
+  HZ_RET_IF(reinterpret_cast<uintptr_t>(bws) % 256 != 0, HZ_ERR_WORKSPACE,
+            "batched workspace must be 256-byte aligned");
+  if (max_iter <= 0) {
+    for (int j = 0; j < n; ++j)
+      if (jobs[j].last_obj) HZ_TRY(fill<float>((float*)jobs[j].last_obj, 1, 0.f, s));
+    return HZ_OK;
+  }
+  // workgroups: every job's share of the co-resident capacity, proportional
+  // to its cells, at least one
+  size_t lds = 0;
+  for (auto& b : J) lds = std::max(lds, sweep_lds_of(b.L));
+  int cap = 0;
+  {
+    const int rc = sweep_multi(J[0].L, lds, nullptr, nullptr, n, 0, &cap, s);
+    HZ_RET_IF(rc != HZ_OK, HZ_ERR_UNSUPPORTED, "the multi-job sweep does not fit (%zu bytes of LDS)",
+              lds);
+  }
+  if (g_tune[HZ_TUNE_SWEEP_WG] > 0) cap = std::min(cap, g_tune[HZ_TUNE_SWEEP_WG]);
+  HZ_RET_IF(n > cap, HZ_ERR_UNSUPPORTED, "%d jobs need at least %d co-resident workgroups (%d)", n,
+            n, cap);
+  {
+    int total = 0;
+    for (int j = 0; j < n; ++j) {
+      J[j].G = std::max(1, (int)((long long)cap * J[j].L.n / std::max(1LL, cells)));
+      total += J[j].G;
+    }
+    while (total > cap) {  // trim the largest shares
+      int jm = 0;
+      for (int j = 1; j < n; ++j)
+        if (J[j].G > J[jm].G) jm = j;
+      --J[jm].G;
+      --total;
+    }
+  }
+  Carve c(bws);
+  SweepArgs* d_args = c.take<SweepArgs>((size_t)max_iter * n);
+  int* d_off = c.take<int>(n + 1);
+  Ctrl** d_ctrls = c.take<Ctrl*>(n);
+  int* d_all = c.take<int>(64);
+
+  std::vector<SweepArgs> h_args((size_t)max_iter * n);
+  std::vector<int> h_off(n + 1, 0);
+  std::vector<Ctrl*> h_ctrls(n);
+  for (int j = 0; j < n; ++j) {
+    hz_job& h = jobs[j];
+    BatchJob& b = J[j];
+    Layout& L = b.L;
+    const float* Z = static_cast<const float*>(h.Z_norm);
+    float* R = static_cast<float*>(h.R);
+    float* O = static_cast<float*>(h.O);
+    float* E = static_cast<float*>(h.E);
+    const float* Pr = static_cast<const float*>(h.Pr_b);
+    const float* th = static_cast<const float*>(h.theta);
+    float* lo = static_cast<float*>(h.last_obj);
+    hipLaunchKernelGGL(ctrl_reset_kernel, dim3(1), dim3(64), 0, s, L.ctrl);
+    HZ_LAUNCHED("ctrl_reset");
+    HZ_TRY(fill<int>(L.counters, L.pd.nblk + 6, 0, s));
+    HZ_TRY(fill<int>(L.ycnt, L.K, 0, s));
+    if (lo) HZ_TRY(fill<float>(lo, 1, 0.f, s));
+    b.epi = make_epi<float>(L, O, E, Pr, th, h.sigma, h.tol, h.stabilized, lo);
+    b.cs = CentSArgs{};
+    b.cs.Z = Z;
+    b.cs.R = R;
+    b.cs.Ypart = reinterpret_cast<float*>(L.Ypart);
+    b.cs.ctrl = L.ctrl;
+    b.cs.n = L.n;
+    b.cs.d = L.d;
+    b.cs.K = L.K;
+    b.ya.Ypart = reinterpret_cast<const float*>(L.Ypart);
+    b.ya.Y = reinterpret_cast<float*>(L.Y);
+    b.ya.YT = reinterpret_cast<float*>(L.YT);
+    b.ya.Yslice = L.Yslice;
+    b.ya.ycnt = L.ycnt;
+    b.ya.G = L.Gs;
+    b.ya.K = L.K;
+    b.ya.d = L.d;
+    b.ya.Kp = L.Kp;
+    b.ya.setup_block0 = 0;
+    b.ya.epi = b.epi;
+    b.ia = InSumArgs{};
+    b.ia.R = R;
+    b.ia.done = &L.ctrl->done;
+    b.ia.acc_in = L.acc_in;
+    b.ia.nseg = (int)((long long)L.pd.nblk * L.B);
+    b.ia.K = L.K;
+    b.ia.scale_o = L.scale_o;
+    b.sw = make_sweep_args(L, Z, R, O, E, Pr, th, h.sigma, h.tol, h.stabilized, lo);
+    for (int it = 0; it < max_iter; ++it) {
+      SweepArgs a = b.sw;
+      bind_sweep_tab(a, tab_of(L.ct, L.pd, it % L.nord));
+      h_args[(size_t)it * n + j] = a;
+    }
+    h_off[j + 1] = h_off[j] + b.G;
+    h_ctrls[j] = L.ctrl;
+  }
+  HZ_HIP(hipMemcpyAsync(d_args, h_args.data(), h_args.size() * sizeof(SweepArgs),
+                        hipMemcpyHostToDevice, s));
+  HZ_HIP(hipMemcpyAsync(d_off, h_off.data(), h_off.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  HZ_HIP(hipMemcpyAsync(d_ctrls, h_ctrls.data(), h_ctrls.size() * sizeof(Ctrl*),
+                        hipMemcpyHostToDevice, s));
+  const int G = h_off[n];
+
+  SideStream* ss = nullptr;
+  HZ_TRY(side_stream(&ss));
+  const hipStream_t s2 = ss->s;
+  HZ_HIP(hipEventRecord(ss->start, s));
+  HZ_HIP(hipStreamWaitEvent(s2, ss->start, 0));
+  DoneChecks* dc = nullptr;
+  HZ_TRY(checks_init(&dc));
+  for (int it = 0; it < max_iter; ++it) {
+    if (it > 0 && it % 2 == 0) {
+      hipLaunchKernelGGL(all_done_kernel, dim3(1), dim3(64), 0, s, d_ctrls, n, d_all);
+      HZ_LAUNCHED("all_done_kernel");
+      HZ_TRY(checks_push_flag(*dc, d_all, s));
+      int done = 0;
+      HZ_TRY(checks_poll(*dc, 3, &done));
+      if (done) break;
+    }
+    // block orders of the next iterations (caller's stream, ahead of them)
+    bool built = false;
+    for (int j = 0; j < n; ++j) {
+      Layout& L = J[j].L;
+      if (it % L.nord != 0) continue;
+      const int cnt = std::min(L.nord, max_iter - it);
+      ProfScope ps(HZ_PROF_ORDER, s);
+      HZ_TRY(launch_block_order(L.pd, jobs[j].seed + (uint32_t)it, jobs[j].cats, L.pb, L.ct,
+                                L.nch_tmp, nullptr, &L.ctrl->order_err, s, cnt));
+      built = true;
+    }
+    if (built) {
+      HZ_HIP(hipEventRecord(ss->tabs, s));
+      HZ_HIP(hipStreamWaitEvent(s2, ss->tabs, 0));
+    }
+    // side stream: every job's old-row sums and contribution counts, once
+    // the previous sweep has finished with R
+    if (it >= 1) HZ_HIP(hipStreamWaitEvent(s2, ss->freed[(it - 1) & 1], 0));
+    for (int j = 0; j < n; ++j) {
+      BatchJob& b = J[j];
+      Layout& L = b.L;
+      const ChunkTab t = tab_of(L.ct, L.pd, it % L.nord);
+      b.ia.order = t.order;
+      b.ia.seg_ch = t.seg_ch;
+      b.ia.ch_start = t.ch_start;
+      b.ia.ch_len = t.ch_len;
+      b.ia.ch_seg = t.ch_seg;
+      ProfScope ps(HZ_PROF_ORDER, s2);
+      HZ_TRY(fill<long long>(L.acc_in, L.nacc * L.K, 0, s2));
+      HZ_TRY(launch_insum(L, b.ia, s2));
+      hipLaunchKernelGGL(sweep_count_kernel, dim3(L.pd.nblk), dim3(kBlock),
+                         (size_t)L.B * kSweepRep * sizeof(int), s2, &L.ctrl->done, t.seg_ch,
+                         t.ch_seg, L.swcnt, L.B, b.G);
+      HZ_LAUNCHED("sweep_count_kernel");
+    }
+    HZ_HIP(hipEventRecord(ss->ready[it & 1], s2));
+    // caller's stream: every job's centroids (natural cell order)
+    for (int j = 0; j < n; ++j) {
+      BatchJob& b = J[j];
+      Layout& L = b.L;
+      {
+        ProfScope ps(HZ_PROF_CENTROID, s);
+        HZ_TRY(launch_cent_stream(L, b.cs, L.Gs, s));
+      }
+      ProfScope ps(HZ_PROF_YREDUCE, s);
+      hipLaunchKernelGGL(yreduce_kernel<float>, dim3(L.K, kYSlices), dim3(kBlock), 0, s, b.ya);
+      HZ_LAUNCHED("yreduce_kernel");
+      HZ_TRY(launch_split(L, L.ctrl, s));
+    }
+    HZ_HIP(hipStreamWaitEvent(s, ss->ready[it & 1], 0));
+    {
+      ProfScope ps(HZ_PROF_SWEEP, s);
+      const SweepArgs* a = d_args + (size_t)it * n;
+      HZ_TRY(chained_sweep(s, [&]() { return sweep_multi(J[0].L, lds, a, d_off, n, G, nullptr, s); }));
+    }
+    HZ_HIP(hipEventRecord(ss->freed[it & 1], s));
+  }
+  HZ_HIP(hipEventRecord(ss->done, s2));
+  HZ_HIP(hipStreamWaitEvent(s, ss->done, 0));
+  std::vector<Ctrl> hc(n);
+  for (int j = 0; j < n; ++j)
+    HZ_HIP(hipMemcpyAsync(&hc[j], J[j].L.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s));
+  HZ_HIP(hipStreamSynchronize(s));
+  int rc = HZ_OK;
+  for (int j = 0; j < n; ++j) {
+    jobs[j].n_iter = hc[j].n_obj;
+    if (hc[j].order_err) {
+      set_error("job %d: block order: a block-boundary bucket held more than %d keys", j, kBoundCap);
+      rc = HZ_ERR_UNSUPPORTED;
+    } else if (hc[j].pad0) {
+      set_error("job %d: block sweep: a wait for the other workgroups' block sums timed out", j);
+      rc = HZ_ERR_LAUNCH;
+    }
+  }
+  return rc;
+}
+
 template <typename T, int DT>
 static int launch_apply_dt(const Layout& L, const ApplyArgs<T>& a, int grid, size_t lds,
                            hipStream_t s) {
@@ -949,8 +1351,14 @@ static int correction_impl(const hz_dims* dm, const T* X, const T* R, const T* O
   ProfScope ps_all(HZ_PROF_CORRECTION, s);
   T* lam = lam_out ? lam_out : reinterpret_cast<T*>(L.lam);
   const int BK = L.B * L.K;
-  hipLaunchKernelGGL(lambda_kernel<T>, dim3((int)ceil_div(BK, 256)), dim3(256), 0, s, E, O, Nb,
-                     lam, L.B, L.K, p);
+  const int BLK = L.BL * L.K;
+  if (L.C >= 2) {
+    const size_t lds = w_multi_lds(L.BL, L.d);
+    HZ_RET_IF(lds > (size_t)device_lds_limit(), HZ_ERR_UNSUPPORTED,
+              "%d levels x %d PCs: the correction solve needs %zu bytes of LDS", L.BL, L.d, lds);
+  }
+  hipLaunchKernelGGL(lambda_kernel<T>, dim3((int)ceil_div(BLK, 256)), dim3(256), 0, s, E, O, Nb,
+                     lam, L.BL, L.K, p);
   HZ_LAUNCHED("lambda_kernel");
   // static chunk tables (cells grouped by batch)
   HZ_TRY(launch_static_chunks(L.n, L.B, L.chunk_p, cat_offsets, cell_indices, L.sp, L.nch_tmp_p, s));
@@ -976,6 +1384,7 @@ static int correction_impl(const hz_dims* dm, const T* X, const T* R, const T* O
   ca.d = L.d;
   ca.K = L.K;
   ca.scale_o = L.scale_o;
+  ca.lv = group_levels_of(L);  // R sums per group
   {
     ProfScope ps(HZ_PROF_PHI, s);
     bool done_mfma = false;
@@ -1001,6 +1410,7 @@ static int correction_impl(const hz_dims* dm, const T* X, const T* R, const T* O
         cm.d = L.d;
         cm.K = L.K;
         cm.scale_o = L.scale_o;
+        cm.lv = group_levels_of(L);
         HZ_TRY(launch_cent_mfma_f32(L, cm, (int)L.spd.maxch, s));
         done_mfma = true;
       }
@@ -1014,7 +1424,25 @@ static int correction_impl(const hz_dims* dm, const T* X, const T* R, const T* O
                        reinterpret_cast<const T*>(L.phipart), L.sp.seg_ch, L.Phi, L.B, L.K, L.d);
     HZ_LAUNCHED("phi_reduce_kernel");
   }
-  {
+  if (L.C >= 2) {
+    WMArgs<T> wm;
+    wm.Phi = L.Phi;
+    wm.rsum = L.rsum;
+    wm.inv_scale = 1.0 / L.scale_o;
+    wm.lam = lam;
+    wm.glev = L.glev;
+    wm.Wl = L.Wl;
+    wm.Wt = reinterpret_cast<T*>(L.Wt);
+    wm.Bg = L.B;
+    wm.BL = L.BL;
+    wm.C = L.C;
+    wm.K = L.K;
+    wm.d = L.d;
+    const size_t lds = w_multi_lds(L.BL, L.d);
+    HZ_TRY(prepare_lds(reinterpret_cast<const void*>(w_multi_kernel<T>), lds));
+    hipLaunchKernelGGL(w_multi_kernel<T>, dim3(L.K), dim3(kBlock), lds, s, wm);
+    HZ_LAUNCHED("w_multi_kernel");
+  } else {
     WArgs<T> wa;
     wa.Phi = L.Phi;
     wa.rsum = L.rsum;
@@ -1244,6 +1672,8 @@ extern "C" {
 
 int hz_version(void) { return 100; }
 
+int hz_release_thread(void) { return release_thread_resources(); }
+
 int hz_set_tuning(int key, int value) {
   HZ_RET_IF(key < 0 || key >= HZ_TUNE_KEYS, HZ_ERR_ARG, "unknown tuning key %d", key);
   HZ_RET_IF(value < 0, HZ_ERR_ARG, "tuning value must be >= 0");
@@ -1416,6 +1846,16 @@ int hz_clustering_loop(const hz_dims* dims, const void* Z_norm, void* R, void* E
                                    sigma, tol, max_iter, seed, stabilized, (double*)last_obj,
                                    n_iter, ws, ws_bytes, s);
   HZ_RET_IF(true, HZ_ERR_ARG, "dtype must be 4 or 8");
+}
+
+size_t hz_batched_workspace_bytes(int n_jobs, int max_iter) {
+  if (n_jobs < 1) return 0;
+  return batched_bytes(n_jobs, max_iter);
+}
+
+int hz_clustering_loop_batched(hz_job* jobs, int n_jobs, void* bws, size_t bws_bytes,
+                               void* stream) {
+  return batched_impl(jobs, n_jobs, bws, bws_bytes, static_cast<hipStream_t>(stream));
 }
 
 int hz_correction(const hz_dims* dims, const void* X, const void* R, const void* O, const void* E,

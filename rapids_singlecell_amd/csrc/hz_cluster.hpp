@@ -39,6 +39,32 @@ struct Ctrl {
 // replicas.
 constexpr int kRep = 8;
 
+// Several covariates (stacked one-hot Phi): the chunk segments are
+// block-major groups of cells that share all levels; every segment adds its
+// rows to, and takes its penalty from, one level of each covariate.  With
+// glev == nullptr (one covariate) a segment is its own level and row() is
+// the identity.
+struct Levels {
+  const int* glev;  // Bg x C level ids, or nullptr
+  int C, Bg, BL;    // covariates, groups, levels
+  __device__ __forceinline__ int count() const { return glev ? C : 1; }
+  // accumulator row (block-major levels) of covariate c of segment seg
+  __device__ __forceinline__ long long row(int seg, int c) const {
+    if (!glev) return seg;
+    const int blk = seg / Bg;
+    return (long long)blk * BL + glev[(seg - blk * Bg) * C + c];
+  }
+};
+
+// Penalty of group g at cluster k: the product of its levels' penalties.
+template <typename T>
+__device__ __forceinline__ T group_pen(const T* pen, const Levels& lv, int g, int k, int K) {
+  if (!lv.glev) return pen[(long long)g * K + k];
+  T p = pen[(long long)lv.glev[g * lv.C] * K + k];
+  for (int c = 1; c < lv.C; ++c) p *= pen[(long long)lv.glev[g * lv.C + c] * K + k];
+  return p;
+}
+
 template <typename T>
 struct EpiArgs {
   T* O;
@@ -49,10 +75,12 @@ struct EpiArgs {
   long long* acc_in;   // nseg x K
   long long* acc_out;  // kRep x nseg x K
   long long* obj_acc;  // kRep x 2: k-means error, entropy (fixed point)
-  long long rep_stride;  // nseg x K
+  long long rep_stride;  // nblk x B x K
   Ctrl* ctrl;
   T* last_obj;         // device scalar written after every iteration
-  int B, K, nblk;
+  int B, K, nblk;      // B = levels (rows of O, E, pen)
+  int Bcol;            // levels of covariate 0: their sums are the column sums
+  Levels lv;
   T sigma, tol;
   int stabilized;
   double inv_scale_o, inv_scale_obj;
@@ -125,7 +153,7 @@ __device__ T block_update(const EpiArgs<T>& a, unsigned char* smem, int smem_byt
     __syncthreads();
     for (int kk = tid; kk < ks; kk += nt) {
       double co = 0.0, ci = 0.0;
-      for (int b = 0; b < B; ++b) {
+      for (int b = 0; b < a.Bcol; ++b) {
         co += s_do[b * ks + kk];
         ci += s_di[b * ks + kk];
       }
@@ -220,6 +248,17 @@ __device__ __forceinline__ bool last_to_arrive(int* counter, int* s_flag) {
   return *s_flag != 0;
 }
 
+// last_to_arrive over `n` workgroups (a part of the grid).
+__device__ __forceinline__ bool last_to_arrive_of(int* counter, int n, int* s_flag) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int prev = __hip_atomic_fetch_add(counter, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    *s_flag = (prev == n - 1) ? 1 : 0;
+  }
+  __syncthreads();
+  return *s_flag != 0;
+}
+
 __device__ __forceinline__ void reset_counter(int* counter) {
   if (threadIdx.x == 0) __hip_atomic_store(counter, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -264,6 +303,7 @@ struct CentArgs {
   const Ctrl* ctrl;
   int nseg, cpw, n, d, K;
   double scale_o;
+  Levels lv;  // accumulator rows of each segment
 };
 
 // Y_partial = R^T Z over a contiguous range of chunks (64-row tiles staged in
@@ -359,7 +399,9 @@ __global__ __launch_bounds__(kBlock) void centroid_kernel(CentArgs<T> a) {
 #pragma unroll
           for (int i = 0; i < KT; ++i) {
             const int k = kg * KT + i;
-            if (k < K) fixed_add(&a.acc_in[(long long)seg * K + k], (double)acc[i][j], a.scale_o);
+            if (k < K)
+              for (int cv = 0; cv < a.lv.count(); ++cv)
+                fixed_add(&a.acc_in[a.lv.row(seg, cv) * K + k], (double)acc[i][j], a.scale_o);
             acc[i][j] = T(0);
           }
         }
@@ -386,33 +428,43 @@ struct YRedArgs {
   const T* Ypart;
   T* Y;
   T* YT;
+  double* Yslice;  // gridDim.y x K x d: per-slice sums of the partials
+  int* ycnt;       // K arrival counters, zero between launches
   int G, K, d, Kp;
   int setup_block0;  // remove block 0's old rows from O/E and compute its penalty
   EpiArgs<T> epi;
 };
 
 constexpr int kYRedLds = 40960;
+constexpr int kYSlices = 8;  // workgroups per centroid row (gridDim.y)
 
-// One workgroup per centroid row k: Y[k] = sum of the G partials (4 waves split
-// the partials, lanes cover the d columns), then the row's L2 normalisation
-// into column k of YT.  Workgroup 0 also prepares block 0 (its old O
-// contribution comes from the centroid pass, an earlier launch).
+// Workgroup (k, slice): the sum of its slice of the G partials of centroid row
+// k (4 waves split the partials, lanes cover the d columns) into Yslice.  The
+// last slice of row k to arrive adds the slices in slice order (the result
+// does not depend on arrival order) and L2-normalises the row into column k
+// of YT.  Workgroup (0, 0) also prepares block 0 (its old O contribution comes
+// from the centroid pass, an earlier launch).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void yreduce_kernel(YRedArgs<T> a) {
   __shared__ double red[kBlock / kWave];
   __shared__ __align__(16) unsigned char smem[kYRedLds];
+  __shared__ int s_last;
   if (a.epi.ctrl->done) return;
+  const int nwg = gridDim.x * gridDim.y;
+  const int wid = blockIdx.y * gridDim.x + blockIdx.x;
   // clear this iteration's block accumulators (read by earlier launches only)
   {
     const long long n_out = (long long)kRep * a.epi.rep_stride;
-    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n_out;
-         e += (long long)gridDim.x * blockDim.x)
+    for (long long e = (long long)wid * blockDim.x + threadIdx.x; e < n_out;
+         e += (long long)nwg * blockDim.x)
       a.epi.acc_out[e] = 0;
-    if (blockIdx.x == 0 && threadIdx.x < 2 * kRep) a.epi.obj_acc[threadIdx.x] = 0;
+    if (wid == 0 && threadIdx.x < 2 * kRep) a.epi.obj_acc[threadIdx.x] = 0;
   }
-  const int k = blockIdx.x;
+  const int k = blockIdx.x, sl = blockIdx.y, ns = gridDim.y;
   const int d = a.d;
   const long long KD = (long long)a.K * d;
+  const int per = (a.G + ns - 1) / ns;
+  const int g0 = sl * per, g1 = min(a.G, g0 + per);
   const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
   double* part = reinterpret_cast<double*>(smem);  // 4 waves x d
   for (int c = lane; c < d; c += kWave) {
@@ -421,15 +473,15 @@ __global__ __launch_bounds__(kBlock) void yreduce_kernel(YRedArgs<T> a) {
     double s[16];
 #pragma unroll
     for (int u = 0; u < 16; ++u) s[u] = 0.0;
-    int g = w;
-    for (; g + 60 < a.G; g += 64) {
+    int g = g0 + w;
+    for (; g + 60 < g1; g += 64) {
       T v[16];
 #pragma unroll
       for (int u = 0; u < 16; ++u) v[u] = src[(long long)(g + 4 * u) * KD];
 #pragma unroll
       for (int u = 0; u < 16; ++u) s[u] += (double)v[u];
     }
-    for (; g < a.G; g += 4) s[0] += (double)src[(long long)g * KD];
+    for (; g < g1; g += 4) s[0] += (double)src[(long long)g * KD];
 #pragma unroll
     for (int h = 8; h > 0; h >>= 1)
 #pragma unroll
@@ -437,23 +489,41 @@ __global__ __launch_bounds__(kBlock) void yreduce_kernel(YRedArgs<T> a) {
     part[w * d + c] = s[0];
   }
   __syncthreads();
-  // row sum of squares in T, as l2_row_normalize_kernel
-  T ss = T(0);
-  T yv[2] = {T(0), T(0)};
-  if (w == 0) {
-    for (int c = lane, u = 0; c < d; c += kWave, ++u) {
-      double v = part[c] + part[d + c] + part[2 * d + c] + part[3 * d + c];
-      T t = T(v);
-      if (u < 2) yv[u] = t;
-      a.Y[(long long)k * d + c] = t;
-      ss += t * t;
-    }
-    ss = wave_sum(ss);
-    T inv = T(1) / sqrt(ss);
-    if (!(inv <= T(1e12))) inv = T(1e12);
-    for (int c = lane, u = 0; c < d; c += kWave, ++u) a.YT[(long long)c * a.Kp + k] = yv[u] * inv;
+  if (w == 0)
+    for (int c = lane; c < d; c += kWave)
+      a.Yslice[((long long)sl * a.K + k) * d + c] =
+          part[c] + part[d + c] + part[2 * d + c] + part[3 * d + c];
+  // arrival of this slice of row k: the writes above are made visible to the
+  // device before the count, and the last slice reads the others after it
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int prev = __hip_atomic_fetch_add(&a.ycnt[k], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s_last = (prev == ns - 1) ? 1 : 0;
   }
-  if (k == 0) {
+  __syncthreads();
+  if (s_last) {
+    __threadfence();
+    // row sum of squares in T, as l2_row_normalize_kernel
+    if (w == 0) {
+      T ss = T(0);
+      T yv[2] = {T(0), T(0)};
+      for (int c = lane, u = 0; c < d; c += kWave, ++u) {
+        double v = 0.0;
+        for (int q = 0; q < ns; ++q) v += a.Yslice[((long long)q * a.K + k) * d + c];
+        T t = T(v);
+        if (u < 2) yv[u] = t;
+        a.Y[(long long)k * d + c] = t;
+        ss += t * t;
+      }
+      ss = wave_sum(ss);
+      T inv = T(1) / sqrt(ss);
+      if (!(inv <= T(1e12))) inv = T(1e12);
+      for (int c = lane, u = 0; c < d; c += kWave, ++u) a.YT[(long long)c * a.Kp + k] = yv[u] * inv;
+      if (lane == 0) __hip_atomic_store(&a.ycnt[k], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+  if (k == 0 && sl == 0) {
     // zero padding columns K..Kp-1 of YT
     for (int e = threadIdx.x; e < d * (a.Kp - a.K); e += blockDim.x) {
       const int c = e / (a.Kp - a.K), kk = a.K + e % (a.Kp - a.K);
@@ -505,7 +575,7 @@ __global__ __launch_bounds__(kBlock) void row_kernel(RowArgs<T> a) {
   Ctrl* ctrl = a.epi.ctrl;
   if (ctrl->done) return;
   const int tid = threadIdx.x, rg = tid >> 4, cg = tid & 15;
-  const int d = a.d, K = a.K, B = a.epi.B;
+  const int d = a.d, K = a.K, B = a.epi.lv.Bg;
   T* YTs = reinterpret_cast<T*>(smem);
   T* ZTs = YTs + (size_t)d * Kp;
   double* dred = reinterpret_cast<double*>(ZTs + (size_t)d * kZtStride);
@@ -529,7 +599,7 @@ __global__ __launch_bounds__(kBlock) void row_kernel(RowArgs<T> a) {
 #pragma unroll
     for (int q = 0; q < CT; ++q) {
       const int kk = cg * CT + q;
-      pv[q] = (kk < K) ? (a.pen ? a.pen[(long long)b * K + kk] : T(1)) : T(0);
+      pv[q] = (kk < K) ? (a.pen ? group_pen(a.pen, a.epi.lv, b, kk, K) : T(1)) : T(0);
     }
     double km = 0.0, ent = 0.0;
     double dO[CT];
@@ -635,7 +705,9 @@ __global__ __launch_bounds__(kBlock) void row_kernel(RowArgs<T> a) {
     for (int k = tid; k < K; k += kBlock) {
       double t = 0.0;
       for (int ww = 0; ww < kBlock / kWave; ++ww) t += dred[ww * Kp + k];
-      sink += fixed_add_wait(&a.acc_out[rep * a.epi.rep_stride + (long long)seg * K + k], t, a.scale_o);
+      for (int cv = 0; cv < a.epi.lv.count(); ++cv)
+        sink += fixed_add_wait(&a.acc_out[rep * a.epi.rep_stride + a.epi.lv.row(seg, cv) * K + k],
+                               t, a.scale_o);
     }
     km = block_sum(km, red);
     ent = block_sum(ent, red);
@@ -652,10 +724,10 @@ __global__ __launch_bounds__(kBlock) void row_kernel(RowArgs<T> a) {
     T obj = block_update(e, smem, a.smem_bytes, 0, 0, kUpdOverwrite | kUpdObjInit, red);
     if (tid == 0 && a.obj_out) *a.obj_out = obj;
   } else if (a.block + 1 < e.nblk) {
-    block_update(e, smem, a.smem_bytes, a.seg_base, a.seg_base + B, kUpdOut | kUpdIn | kUpdPen,
-                 red);
+    block_update(e, smem, a.smem_bytes, a.block * e.B, (a.block + 1) * e.B,
+                 kUpdOut | kUpdIn | kUpdPen, red);
   } else {
-    block_update(e, smem, a.smem_bytes, a.seg_base, 0, kUpdOut | kUpdObj, red);
+    block_update(e, smem, a.smem_bytes, a.block * e.B, 0, kUpdOut | kUpdObj, red);
   }
   reset_counter(a.counter);
 }

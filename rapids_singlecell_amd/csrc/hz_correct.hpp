@@ -257,6 +257,113 @@ __global__ __launch_bounds__(kBlock) void w_kernel(WArgs<T> a) {
   }
 }
 
+// ---------------------------------------------------------------- several covariates
+
+template <typename T>
+struct WMArgs {
+  const double* Phi;      // Bg x K x d: R_k^T X over the cells of each group
+  const long long* rsum;  // Bg x K fixed point: R sums of each group
+  double inv_scale;
+  const T* lam;           // BL x K ridge penalties of the levels
+  const int* glev;        // Bg x C level ids of each group
+  double* Wl;             // K x (BL + 1) x d: solutions, row 0 zeroed
+  T* Wt;                  // Bg x K x d: each group's term, sum of its levels' rows
+  int Bg, BL, C, K, d;
+};
+
+__host__ __device__ inline size_t w_multi_lds(int BL, int d) {
+  const size_t n = (size_t)BL + 1;
+  return (n * n + n * (size_t)d) * sizeof(double);
+}
+
+// Stacked one-hot Phi, one workgroup per cluster k:
+//   A = [1 | Phi]^T diag(R_k) [1 | Phi] + diag(0, lambda_k),
+//   rhs = [1 | Phi]^T diag(R_k) X,  W = A^{-1} rhs (FP64 Cholesky), W[0] = 0.
+// The Gram entries are sums of the groups' R sums: every group adds to the
+// intercept, to its levels' row-0 and diagonal entries and to the
+// cross-covariate entries of its level pairs.  They are accumulated as exact
+// fixed-point integers (order-independent) and converted once.  The
+// right-hand side adds the groups' Phi rows in group order.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void w_multi_kernel(WMArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int k = blockIdx.x;
+  const int n = a.BL + 1, d = a.d, C = a.C, Bg = a.Bg, K = a.K;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  unsigned long long* Aq = reinterpret_cast<unsigned long long*>(smem);
+  double* A = reinterpret_cast<double*>(smem);  // the same storage, converted in place
+  double* rhs = A + (size_t)n * n;              // n x d
+  for (int e = tid; e < n * n; e += nt) Aq[e] = 0ull;
+  for (int e = tid; e < n * d; e += nt) rhs[e] = 0.0;
+  __syncthreads();
+  for (int g = tid; g < Bg; g += nt) {
+    const unsigned long long q = (unsigned long long)a.rsum[(long long)g * K + k];
+    if (q == 0ull) continue;
+    atomicAdd(&Aq[0], q);
+    for (int c = 0; c < C; ++c) {
+      const int l = 1 + a.glev[g * C + c];
+      atomicAdd(&Aq[l], q);
+      atomicAdd(&Aq[(size_t)l * n], q);
+      for (int c2 = 0; c2 < C; ++c2) atomicAdd(&Aq[(size_t)l * n + 1 + a.glev[g * C + c2]], q);
+    }
+  }
+  // right-hand side: groups in order; a thread owns one column
+  for (int col = tid; col < d; col += nt) {
+    for (int g = 0; g < Bg; ++g) {
+      const double v = a.Phi[((long long)g * K + k) * d + col];
+      rhs[col] += v;
+      for (int c = 0; c < C; ++c) rhs[(size_t)(1 + a.glev[g * C + c]) * d + col] += v;
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < n * n; e += nt) {
+    const int i = e / n, j = e - i * n;
+    double v = (double)(long long)Aq[e] * a.inv_scale;
+    if (i == j && i > 0) v += (double)a.lam[(long long)(i - 1) * K + k];
+    A[e] = v;
+  }
+  __syncthreads();
+  // Cholesky A = L L^T in place (lower triangle), column by column
+  for (int j = 0; j < n; ++j) {
+    if (tid == 0) {
+      double s = A[j * n + j];
+      for (int t = 0; t < j; ++t) s -= A[j * n + t] * A[j * n + t];
+      A[j * n + j] = sqrt(s);
+    }
+    __syncthreads();
+    const double dj = A[j * n + j];
+    for (int i = j + 1 + tid; i < n; i += nt) {
+      double s = A[i * n + j];
+      for (int t = 0; t < j; ++t) s -= A[i * n + t] * A[j * n + t];
+      A[i * n + j] = s / dj;
+    }
+    __syncthreads();
+  }
+  // forward then backward substitution, one right-hand-side column per thread
+  for (int col = tid; col < d; col += nt) {
+    for (int i = 0; i < n; ++i) {
+      double s = rhs[i * d + col];
+      for (int t = 0; t < i; ++t) s -= A[i * n + t] * rhs[t * d + col];
+      rhs[i * d + col] = s / A[i * n + i];
+    }
+    for (int i = n - 1; i >= 0; --i) {
+      double s = rhs[i * d + col];
+      for (int t = i + 1; t < n; ++t) s -= A[t * n + i] * rhs[t * d + col];
+      rhs[i * d + col] = s / A[i * n + i];
+    }
+  }
+  __syncthreads();
+  double* Wk = a.Wl + (long long)k * n * d;
+  for (int e = tid; e < n * d; e += nt) Wk[e] = (e < d) ? 0.0 : rhs[e];
+  // each group's term: the sum of its levels' rows (in covariate order)
+  for (long long e = tid; e < (long long)Bg * d; e += nt) {
+    const int g = (int)(e / d), col = (int)(e - (long long)g * d);
+    double w = 0.0;
+    for (int c = 0; c < C; ++c) w += rhs[(size_t)(1 + a.glev[g * C + c]) * d + col];
+    a.Wt[((long long)g * K + k) * d + col] = T(w);
+  }
+}
+
 template <typename T>
 struct ApplyArgs {
   const T* X;

@@ -13,6 +13,12 @@ namespace hz {
 
 struct Layout {
   int n, d, K, B, bs, esz;
+  // several covariates: B counts the groups of cells sharing all levels, BL
+  // the levels (rows of O and E), BL0 those of covariate 0; one covariate:
+  // C = 1, BL = BL0 = B, glev = nullptr
+  int C, BL, BL0;
+  const int* glev;
+  long long nacc;  // accumulator rows: nblk x BL
   int CT, Kp, DT, Dp;
   int chunk, chunk_p, G, cpw;
   // matrix-core path (FP32 storage)
@@ -41,6 +47,8 @@ struct Layout {
   float* swtrash;     // 256 floats: stores of the sweep's padded lanes
   int Gs = 1;         // workgroups of the streaming centroid pass
   void* Ypart;
+  double* Yslice;  // kYSlices x K x d: centroid reduction slices
+  int* ycnt;       // K arrival counters of the centroid reduction
   void* Y;
   void* YT;
   void* pen;
@@ -50,6 +58,7 @@ struct Layout {
   long long* rsum;
   void* lam;
   void* Wt;
+  double* Wl;  // several covariates: per-level solutions, K x (BL + 1) x d
   size_t bytes;
   double scale_o, scale_obj;
 };
@@ -72,6 +81,14 @@ int launch_row_mfma_bf16(const Layout& L, const RowMArgs& a, int grid, hipStream
 // a == nullptr: choose the grid only; HZ_ERR_UNSUPPORTED when not co-resident
 int launch_sweep_f32(const Layout& L, const SweepArgs* a, int* G, hipStream_t s);
 int launch_sweep_bf16(const Layout& L, const SweepArgs* a, int* G, hipStream_t s);
+// several jobs of one kernel shape in one cooperative sweep launch; d_jobs ==
+// nullptr: only the co-resident capacity (*cap workgroups) for `lds` bytes
+int sweep_multi_f32(const Layout& L, size_t lds, const SweepArgs* d_jobs, const int* d_off,
+                    int n_jobs, int G, int* cap, hipStream_t s);
+int sweep_multi_bf16(const Layout& L, size_t lds, const SweepArgs* d_jobs, const int* d_off,
+                     int n_jobs, int G, int* cap, hipStream_t s);
+size_t sweep_lds_f32(const Layout& L);
+size_t sweep_lds_bf16(const Layout& L);
 int launch_cent_stream_f32(const Layout& L, const CentSArgs& a, int grid, hipStream_t s);
 int launch_cent_stream_bf16(const Layout& L, const CentSArgs& a, int grid, hipStream_t s);
 int launch_cent_mfma_f32(const Layout& L, const CentMArgs& a, int grid, hipStream_t s);
@@ -85,6 +102,14 @@ inline int launch_row_mfma(const Layout& L, const RowMArgs& a, int grid, hipStre
 }
 inline int launch_sweep(const Layout& L, const SweepArgs* a, int* G, hipStream_t s) {
   return L.prec == HZ_PREC_SPLIT_BF16 ? launch_sweep_bf16(L, a, G, s) : launch_sweep_f32(L, a, G, s);
+}
+inline int sweep_multi(const Layout& L, size_t lds, const SweepArgs* d_jobs, const int* d_off,
+                       int n_jobs, int G, int* cap, hipStream_t s) {
+  return L.prec == HZ_PREC_SPLIT_BF16 ? sweep_multi_bf16(L, lds, d_jobs, d_off, n_jobs, G, cap, s)
+                                      : sweep_multi_f32(L, lds, d_jobs, d_off, n_jobs, G, cap, s);
+}
+inline size_t sweep_lds_of(const Layout& L) {
+  return L.prec == HZ_PREC_SPLIT_BF16 ? sweep_lds_bf16(L) : sweep_lds_f32(L);
 }
 inline int launch_cent_stream(const Layout& L, const CentSArgs& a, int grid, hipStream_t s) {
   return L.prec == HZ_PREC_SPLIT_BF16 ? launch_cent_stream_bf16(L, a, grid, s)

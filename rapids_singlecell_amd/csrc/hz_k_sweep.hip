@@ -43,6 +43,46 @@ static int launch_sw(const Layout& L, const SweepArgs* a, int* Gp, hipStream_t s
   return HZ_OK;
 }
 
+// Several jobs in one cooperative launch (sweep_multi_kernel).  d_jobs ==
+// nullptr: only report the co-resident capacity (*cap workgroups) for `lds`.
+template <int PREC, int NT, int SM>
+static int launch_swm(size_t lds, const SweepArgs* d_jobs, const int* d_off, int n_jobs, int G,
+                      int* cap, hipStream_t s) {
+  auto kern = sweep_multi_kernel<PREC, NT, SM>;
+  if (prepare_lds(reinterpret_cast<const void*>(kern), lds) != HZ_OK) return HZ_ERR_UNSUPPORTED;
+  int per_cu = 0;
+  HZ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern),
+                                                      kBlock, lds));
+  if (per_cu < 1) return HZ_ERR_UNSUPPORTED;
+  if (cap) *cap = per_cu * device_cu_count();
+  if (!d_jobs) return HZ_OK;
+  if (tune_value(HZ_TUNE_SWEEP_PLAIN)) {
+    hipLaunchKernelGGL(kern, dim3((unsigned)G), dim3(kBlock), lds, s, d_jobs, d_off, n_jobs);
+    HZ_LAUNCHED("sweep_multi_kernel");
+    return HZ_OK;
+  }
+  void* args[] = {const_cast<SweepArgs**>(&d_jobs), const_cast<int**>(&d_off), &n_jobs};
+  hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kern), dim3((unsigned)G),
+                                            dim3(kBlock), args, (unsigned)lds, s);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    set_error("cooperative launch of the multi-job sweep failed: %s", hipGetErrorString(e));
+    return HZ_ERR_LAUNCH;
+  }
+  return HZ_OK;
+}
+
+template <int PREC, int SM>
+static int launch_swm_sm(const Layout& L, size_t lds, const SweepArgs* d_jobs, const int* d_off,
+                         int n_jobs, int G, int* cap, hipStream_t s) {
+  switch (L.NT) {
+    case 4: return launch_swm<PREC, 4, SM>(lds, d_jobs, d_off, n_jobs, G, cap, s);
+    case 7: return launch_swm<PREC, 7, SM>(lds, d_jobs, d_off, n_jobs, G, cap, s);
+    case 8: return launch_swm<PREC, 8, SM>(lds, d_jobs, d_off, n_jobs, G, cap, s);
+    default: return launch_swm<PREC, 16, SM>(lds, d_jobs, d_off, n_jobs, G, cap, s);
+  }
+}
+
 template <int PREC, int SM>
 static int launch_sw_sm(const Layout& L, const SweepArgs* a, int* G, hipStream_t s) {
   switch (L.NT) {
@@ -55,6 +95,29 @@ static int launch_sw_sm(const Layout& L, const SweepArgs* a, int* G, hipStream_t
 
 
 #if HZ_UNIT_PREC == 0
+size_t sweep_lds_f32(const Layout& L) {
+  switch (row_sm(L)) {
+    case 4: return sweep_lds<kPrecF32, 4>(L.Kp, L.B, L.K, L.pd.nblk);
+    case 8: return sweep_lds<kPrecF32, 8>(L.Kp, L.B, L.K, L.pd.nblk);
+    case 13: return sweep_lds<kPrecF32, 13>(L.Kp, L.B, L.K, L.pd.nblk);
+    case 16: return sweep_lds<kPrecF32, 16>(L.Kp, L.B, L.K, L.pd.nblk);
+    case 25: return sweep_lds<kPrecF32, 25>(L.Kp, L.B, L.K, L.pd.nblk);
+    default: return sweep_lds<kPrecF32, 32>(L.Kp, L.B, L.K, L.pd.nblk);
+  }
+}
+
+int sweep_multi_f32(const Layout& L, size_t lds, const SweepArgs* d_jobs, const int* d_off,
+                    int n_jobs, int G, int* cap, hipStream_t s) {
+  switch (row_sm(L)) {
+    case 4: return launch_swm_sm<kPrecF32, 4>(L, lds, d_jobs, d_off, n_jobs, G, cap, s);
+    case 8: return launch_swm_sm<kPrecF32, 8>(L, lds, d_jobs, d_off, n_jobs, G, cap, s);
+    case 13: return launch_swm_sm<kPrecF32, 13>(L, lds, d_jobs, d_off, n_jobs, G, cap, s);
+    case 16: return launch_swm_sm<kPrecF32, 16>(L, lds, d_jobs, d_off, n_jobs, G, cap, s);
+    case 25: return launch_swm_sm<kPrecF32, 25>(L, lds, d_jobs, d_off, n_jobs, G, cap, s);
+    default: return launch_swm_sm<kPrecF32, 32>(L, lds, d_jobs, d_off, n_jobs, G, cap, s);
+  }
+}
+
 int launch_sweep_f32(const Layout& L, const SweepArgs* a, int* G, hipStream_t s) {
   switch (row_sm(L)) {
     case 4: return launch_sw_sm<kPrecF32, 4>(L, a, G, s);
@@ -66,6 +129,25 @@ int launch_sweep_f32(const Layout& L, const SweepArgs* a, int* G, hipStream_t s)
   }
 }
 #else
+size_t sweep_lds_bf16(const Layout& L) {
+  switch (row_sm(L)) {
+    case 1: return sweep_lds<kPrecBF16, 1>(L.Kp, L.B, L.K, L.pd.nblk);
+    case 2: return sweep_lds<kPrecBF16, 2>(L.Kp, L.B, L.K, L.pd.nblk);
+    case 3: return sweep_lds<kPrecBF16, 3>(L.Kp, L.B, L.K, L.pd.nblk);
+    default: return sweep_lds<kPrecBF16, 4>(L.Kp, L.B, L.K, L.pd.nblk);
+  }
+}
+
+int sweep_multi_bf16(const Layout& L, size_t lds, const SweepArgs* d_jobs, const int* d_off,
+                     int n_jobs, int G, int* cap, hipStream_t s) {
+  switch (row_sm(L)) {
+    case 1: return launch_swm_sm<kPrecBF16, 1>(L, lds, d_jobs, d_off, n_jobs, G, cap, s);
+    case 2: return launch_swm_sm<kPrecBF16, 2>(L, lds, d_jobs, d_off, n_jobs, G, cap, s);
+    case 3: return launch_swm_sm<kPrecBF16, 3>(L, lds, d_jobs, d_off, n_jobs, G, cap, s);
+    default: return launch_swm_sm<kPrecBF16, 4>(L, lds, d_jobs, d_off, n_jobs, G, cap, s);
+  }
+}
+
 int launch_sweep_bf16(const Layout& L, const SweepArgs* a, int* G, hipStream_t s) {
   switch (row_sm(L)) {
     case 1: return launch_sw_sm<kPrecBF16, 1>(L, a, G, s);

@@ -114,7 +114,7 @@ __global__ __launch_bounds__(kBlock) void row_mfma_kernel(RowMArgs a) {
   if (ctrl->done) return;
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
   const int lr = lane & 15, lg = lane >> 4;  // row/col inside a 16-wide fragment, k group
-  const int d = a.d, K = a.K, Kp = a.Kp, B = a.epi.B;
+  const int d = a.d, K = a.K, Kp = a.Kp, B = a.epi.lv.Bg;
   const int nch_lo = a.seg_ch[a.seg_base];
   const int nch_hi = a.seg_ch[a.seg_base + B];
   const int c0 = nch_lo + blockIdx.x * (kBlock / kWave);
@@ -182,7 +182,7 @@ __global__ __launch_bounds__(kBlock) void row_mfma_kernel(RowMArgs a) {
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
       const int k = 16 * t + lr;
-      pv[t] = (k < K) ? (a.pen ? a.pen[(long long)b * K + k] : 1.f) : 0.f;
+      pv[t] = (k < K) ? (a.pen ? group_pen(a.pen, a.epi.lv, b, k, K) : 1.f) : 0.f;
     }
     float dO[NT];
 #pragma unroll
@@ -299,7 +299,8 @@ __global__ __launch_bounds__(kBlock) void row_mfma_kernel(RowMArgs a) {
         float x = 0.f;
         int w2 = w;
         while (w2 < kBlock / kWave && s_seg[w2] == sg) x += s_part[w2++][k];
-        sink += fixed_add_wait(&acc[(long long)sg * K + k], (double)x, a.scale_o);
+        for (int cv = 0; cv < a.epi.lv.count(); ++cv)
+          sink += fixed_add_wait(&acc[a.epi.lv.row(sg, cv) * K + k], (double)x, a.scale_o);
         w = w2;
       }
     }
@@ -321,10 +322,10 @@ __global__ __launch_bounds__(kBlock) void row_mfma_kernel(RowMArgs a) {
     float obj = block_update(e, smem, a.smem_bytes, 0, 0, kUpdOverwrite | kUpdObjInit, red);
     if (tid == 0 && a.obj_out) *a.obj_out = obj;
   } else if (a.block + 1 < e.nblk) {
-    block_update(e, smem, a.smem_bytes, a.seg_base, a.seg_base + B, kUpdOut | kUpdIn | kUpdPen,
-                 red);
+    block_update(e, smem, a.smem_bytes, a.block * e.B, (a.block + 1) * e.B,
+                 kUpdOut | kUpdIn | kUpdPen, red);
   } else {
-    block_update(e, smem, a.smem_bytes, a.seg_base, 0, kUpdOut | kUpdObj, red);
+    block_update(e, smem, a.smem_bytes, a.block * e.B, 0, kUpdOut | kUpdObj, red);
   }
   reset_counter(a.counter);
 }
@@ -345,6 +346,7 @@ struct CentMArgs {
   const Ctrl* ctrl;
   int nseg, cpw, n, d, K;
   double scale_o;
+  Levels lv;  // accumulator rows of each segment
 };
 
 // LDS row stride (elements) congruent to 16 mod 64 so the four row groups of
@@ -513,7 +515,9 @@ __global__ __launch_bounds__(kBlock) void centroid_mfma_kernel(CentMArgs a) {
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int k = 16 * t + 4 * lg + r;
-            if (k < K) fixed_add(&a.acc_in[(long long)seg * K + k], (double)acc[j][r], a.scale_o);
+            if (k < K)
+              for (int cv = 0; cv < a.lv.count(); ++cv)
+                fixed_add(&a.acc_in[a.lv.row(seg, cv) * K + k], (double)acc[j][r], a.scale_o);
           }
           acc[j] = floatx4{0.f, 0.f, 0.f, 0.f};
         }

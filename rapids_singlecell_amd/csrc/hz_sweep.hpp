@@ -218,8 +218,9 @@ __device__ __forceinline__ float exp_rows(float x) {
   return __builtin_fmaf(y, e * kLn2, y);
 }
 
+// The sweep of one job run by `G` workgroups, this one being number `wg`.
 template <int PREC, int NT, int SM>
-__global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
+__device__ __forceinline__ void sweep_body(const SweepArgs& a, const int wg, const int G) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int NW = kBlock / kWave;
   constexpr int NA = (PREC == kPrecF32) ? SM : 8 * SM;
@@ -227,7 +228,7 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
   __shared__ double red[NW];
   __shared__ float s_part[NW][16 * NT];
   __shared__ int s_seg[NW];
-  __shared__ double s_obj2[NW][2];
+  __shared__ long long s_objq[NW][2];
   Ctrl* ctrl = a.ctrl;
   // written only between launches, so the same in every workgroup; an earlier
   // wait that ran out (pad0) ends the remaining launches at once
@@ -236,7 +237,6 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
   const int lr = lane & 15, lg = lane >> 4;
   const int d = a.d, K = a.K, Kp = a.Kp, B = a.B;
   const int BK = B * K;
-  const int G = gridDim.x;
 
   // LDS carve-up
   const size_t yb = align_up(row_mfma_lds<PREC, SM>(Kp), 16);
@@ -291,11 +291,12 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
   const float* YTs = reinterpret_cast<const float*>(smem);
   const __bf16* YHs = reinterpret_cast<const __bf16*>(smem);
   const __bf16* YLs = YHs + (size_t)Kp * dPS;
-  const int rep = blockIdx.x % kSweepRep;
+  const int rep = wg % kSweepRep;
   long long* out_rep = a.acc_out + rep * a.rep_stride;
   int* err = &ctrl->pad0;
   const double inv_o = a.inv_scale_o;
-  float kmf = 0.f, entf = 0.f;
+  // objective terms, fixed point per lane (2^-24 units, see finish_tile)
+  long long kmq = 0, entq = 0;
 
   // this workgroup's contiguous chunk range of block j (may be empty)
   auto wg_range = [&](int j, int& lo, int& hi) __attribute__((always_inline)) {
@@ -308,7 +309,7 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
   for (int j = tid; j < a.nblk; j += kBlock) {
     const int nlo = a.seg_ch[j * B], nhi = a.seg_ch[j * B + B];
     const int per = (nhi - nlo + G - 1) / G;
-    const int lo = nlo + blockIdx.x * per;
+    const int lo = nlo + wg * per;
     const int hi = min(nhi, lo + per);
     s_lohi[4 * j] = lo;
     s_lohi[4 * j + 1] = hi;
@@ -422,6 +423,7 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
   // less than 1e-12 per value).
   auto finish_tile = [&](int b, int my_cell, int len, int r0, const floatx4 (&acc)[NT],
                          float (&dO)[NT]) __attribute__((always_inline)) {
+    float kmf = 0.f, entf = 0.f;
     float lp[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
@@ -473,6 +475,12 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
       kmf = __builtin_fmaf(2.f * m, kv, kmf);
       entf = __builtin_fmaf(m * inv2, __builtin_fmaf(lrow, s2_lane, va), entf);
     }
+    // this tile's terms (|k-means| <= 16, |entropy| <= 4 log K per lane) in
+    // integer units of 2^-24: the sums are then exact and independent of how
+    // the tiles are spread over workgroups and waves
+    constexpr float kObjQ = 16777216.f;
+    kmq += (long long)(int)__builtin_rintf(kmf * kObjQ);
+    entq += (long long)(int)__builtin_rintf(entf * kObjQ);
   };
 
   // wave-carried prefetch of the first chunk of the next block: its table
@@ -814,25 +822,28 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
   // objective terms of this workgroup (kernels_kmeans.cuh:30-95,
   // kernels_clustering.cuh:9-62)
   {
-    const double km = wave_sum((double)kmf);
-    const double ent = wave_sum((double)entf);
+    const long long km = wave_sum(kmq);
+    const long long ent = wave_sum(entq);
     if (lane == 0) {
-      s_obj2[wv][0] = km;
-      s_obj2[wv][1] = ent;
+      s_objq[wv][0] = km;
+      s_objq[wv][1] = ent;
     }
     __syncthreads();
     if (tid == 0) {
-      double km4 = 0.0, ent4 = 0.0;
+      long long km4 = 0, ent4 = 0;
       for (int w = 0; w < NW; ++w) {
-        km4 += s_obj2[w][0];
-        ent4 += s_obj2[w][1];
+        km4 += s_objq[w][0];
+        ent4 += s_objq[w][1];
       }
-      const unsigned long long r0 = fixed_add_wait(&a.obj_acc[2 * rep], km4, a.scale_obj);
-      const unsigned long long r1 = fixed_add_wait(&a.obj_acc[2 * rep + 1], ent4, a.scale_obj);
+      // 2^-24 units to the accumulator's scale (a power of two >= 2^24:
+      // exact for sums below 2^53)
+      const double to_acc = a.scale_obj * 0x1p-24;
+      const unsigned long long r0 = fixed_add_wait(&a.obj_acc[2 * rep], (double)km4, to_acc);
+      const unsigned long long r1 = fixed_add_wait(&a.obj_acc[2 * rep + 1], (double)ent4, to_acc);
       asm volatile("" ::"v"(r0 + r1));
     }
   }
-  if (!last_to_arrive(a.fin, &s_last)) return;
+  if (!last_to_arrive_of(a.fin, G, &s_last)) return;
 
   // last workgroup: final O/E, objective, window-3 test (clustering.cu:375-387)
   double div = 0.0;
@@ -868,6 +879,26 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
     }
     __hip_atomic_store(a.fin, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
+}
+
+// One job: the whole grid.
+template <int PREC, int NT, int SM>
+__global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a) {
+  sweep_body<PREC, NT, SM>(a, blockIdx.x, gridDim.x);
+}
+
+// Several jobs of one kernel shape in one cooperative launch: workgroups
+// off[j] .. off[j+1]-1 run job j's sweep (its arguments in jobs[j]).  The
+// jobs share nothing, so the waits of one job's block exchange are filled by
+// the others' rows on the rest of the device.
+template <int PREC, int NT, int SM>
+__global__ __launch_bounds__(kBlock) void sweep_multi_kernel(const SweepArgs* __restrict__ jobs,
+                                                             const int* __restrict__ off,
+                                                             int n_jobs) {
+  int j = 0;
+  while (j + 1 < n_jobs && (int)blockIdx.x >= off[j + 1]) ++j;
+  const SweepArgs a = jobs[j];
+  sweep_body<PREC, NT, SM>(a, (int)blockIdx.x - off[j], off[j + 1] - off[j]);
 }
 
 }  // namespace hz

@@ -17,7 +17,7 @@ import numpy as np
 import torch
 
 from ... import _native
-from ._helper import _get_batch_codes, _get_theta_array
+from ._helper import _get_batch_codes, _get_covariates, _get_theta_array, _get_theta_levels
 from ._job import HarmonyJob
 
 _SUPPRESS_PENALTY = 1e30
@@ -98,13 +98,19 @@ def harmonize(
     init_centroids=None,
     precision: str = "native",
     record: dict | None = None,
+    multi_covariate: bool = False,
 ) -> torch.Tensor:
     """Integrate data using the Harmony algorithm (device tensors in and out).
 
     Parameters and errors follow the reference ``harmonize``.  Extensions:
     ``init_centroids`` (K x d) replaces the k-means initialiser (parity runs
     inject the same centroids into the oracle); ``precision`` selects the GEMM
-    arithmetic; ``record`` receives objectives and clustering iteration counts.
+    arithmetic; ``record`` receives objectives and clustering iteration counts;
+    ``multi_covariate=True`` with several keys treats each key as a covariate
+    of its own (stacked one-hot Phi: O and E per level, a cell's penalty is
+    the product of its levels' penalties, the ridge correction solves the
+    full normal equations) instead of joining the keys into one categorical.
+    ``theta`` is then a scalar, one value per key or one value per level.
     ``colsum_algo`` is validated and otherwise ignored: column sums are fused
     into the row kernel.
     """
@@ -123,13 +129,24 @@ def harmonize(
     if prec == 1 and Z.dtype != torch.float32:
         raise ValueError("precision='split_bf16' needs float32 data")
 
-    # batch bookkeeping (_harmony/__init__.py:146-152)
-    batch_codes = _get_batch_codes(batch_mat, batch_key)
-    n_batches = batch_codes.cat.categories.size
-    N_b_np = np.asarray(batch_codes.value_counts(sort=False).values, dtype=np_dtype)
-    cats_np = np.asarray(batch_codes.cat.codes.values, dtype=np.int32)
-    if n_cells and cats_np.min() < 0:
-        raise ValueError(f"batch key {batch_key!r} has missing values")
+    multi = bool(multi_covariate) and not isinstance(batch_key, str) and len(batch_key) >= 2
+    group_levels = None
+    n_levels0 = 0
+    if multi:
+        # one covariate per key: cells are grouped by their tuple of levels
+        _, n_levels, group_levels, cats_np = _get_covariates(batch_mat, list(batch_key))
+        n_batches = int(group_levels.shape[0])
+        N_b_np = np.bincount(group_levels[cats_np].ravel(),
+                             minlength=int(sum(n_levels))).astype(np_dtype)
+        n_levels0 = int(n_levels[0])
+    else:
+        # batch bookkeeping (_harmony/__init__.py:146-152)
+        batch_codes = _get_batch_codes(batch_mat, batch_key)
+        n_batches = batch_codes.cat.categories.size
+        N_b_np = np.asarray(batch_codes.value_counts(sort=False).values, dtype=np_dtype)
+        cats_np = np.asarray(batch_codes.cat.codes.values, dtype=np.int32)
+        if n_cells and cats_np.min() < 0:
+            raise ValueError(f"batch key {batch_key!r} has missing values")
 
     # parameters (_harmony/__init__.py:155-200)
     if max_iter_harmony < 1:
@@ -138,7 +155,10 @@ def harmonize(
         n_clusters = int(min(100, n_cells / 30))
         n_clusters = max(n_clusters, 2)
     assert colsum_algo in COLSUM_ALGOS
-    theta_np = _get_theta_array(theta, n_batches, np_dtype)
+    if multi:
+        theta_np = _get_theta_levels(theta, n_levels, np_dtype)
+    else:
+        theta_np = _get_theta_array(theta, n_batches, np_dtype)
     if tau > 0:
         theta_np = theta_np * (1 - np.exp(-N_b_np / (n_clusters * tau)) ** 2)
     theta_np = np.ascontiguousarray(theta_np.astype(np_dtype).ravel())
@@ -172,7 +192,8 @@ def harmonize(
         tol_clustering=tol_clustering, ridge_lambda=ridge_lambda,
         stabilized_penalty=stabilized_penalty, dynamic_lambda=dynamic_lambda, alpha=alpha,
         batch_prune_threshold=batch_prune_threshold, correction_method=correction_method,
-        random_state=random_state, precision=prec)
+        random_state=random_state, precision=prec, group_levels=group_levels,
+        n_levels0=n_levels0)
 
     # initialisation (_harmony/__init__.py:206-217, 292-353)
     Y0 = None
@@ -199,7 +220,8 @@ def harmonize(
     if record is not None:
         record.update(objectives=objectives_harmony, clustering_iters=clustering_iters, R=job.R,
                       E=job.E, O=job.O, Z_norm=job.Z_norm, lambda_kb=job.lam, Y0=job.Y0,
-                      n_clusters=n_clusters, correction_method=correction_method)
+                      n_clusters=n_clusters, correction_method=correction_method,
+                      group_levels=group_levels)
     return job.Z_hat
 
 

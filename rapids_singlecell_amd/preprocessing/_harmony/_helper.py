@@ -60,3 +60,49 @@ def _get_theta_array(theta, n_batches: int, dtype) -> np.ndarray:
             f"Theta array size ({arr.size}) must match number of batches ({n_batches})"
         )
     return arr.ravel()
+
+
+def _get_covariates(batch_mat, keys):
+    """Several covariates, one per key (stacked one-hot Phi).
+
+    Each key is made categorical on its own (pandas category order, as a
+    single key is in ``_get_batch_codes``).  Returns ``levels`` (n_cells x C
+    int32: covariate c's code offset by the levels of the covariates before
+    it), the number of levels per covariate, the groups of cells that share
+    all levels (``group_levels``: n_groups x C in lexicographic order) and each
+    cell's group.  No counterpart in the reference, which joins several keys
+    into one categorical.
+    """
+    codes, n_levels = [], []
+    for k in keys:
+        cat = batch_mat[k].astype("category")
+        c = np.asarray(cat.cat.codes.values, dtype=np.int64)
+        if c.size and c.min() < 0:
+            raise ValueError(f"batch key {k!r} has missing values")
+        codes.append(c)
+        n_levels.append(int(cat.cat.categories.size))
+    offs = np.concatenate([[0], np.cumsum(n_levels)[:-1]])
+    levels = np.stack([c + o for c, o in zip(codes, offs)], axis=1).astype(np.int32)
+    group_levels, group = np.unique(levels, axis=0, return_inverse=True)
+    return (levels, n_levels, np.ascontiguousarray(group_levels, dtype=np.int32),
+            np.asarray(group, dtype=np.int32).ravel())
+
+
+def _get_theta_levels(theta, n_levels, dtype) -> np.ndarray:
+    """Theta per level for several covariates: a scalar, one value per
+    covariate (broadcast to its levels) or one value per level."""
+    dtype = np.dtype(dtype)
+    total = int(sum(n_levels))
+    if isinstance(theta, (float, int)):
+        return np.ones(total, dtype=dtype) * float(theta)
+    if hasattr(theta, "detach") and hasattr(theta, "cpu"):
+        theta = theta.detach().cpu().numpy()
+    arr = np.asarray(theta, dtype=dtype).ravel()
+    if arr.size == len(n_levels):
+        return np.repeat(arr, n_levels).astype(dtype)
+    if arr.size == total:
+        return arr
+    raise ValueError(
+        f"Theta needs one value, one per covariate ({len(n_levels)}) or one per level ({total}), "
+        f"got {arr.size}"
+    )

@@ -30,7 +30,11 @@ class HarmonyJob:
                  stabilized_penalty: bool = True, dynamic_lambda: bool = True,
                  alpha: float = 0.2, batch_prune_threshold: float | None = 1e-5,
                  correction_method: str = "batched", random_state: int = 0,
-                 precision: int = 0, stream: torch.cuda.Stream | None = None):
+                 precision: int = 0, stream: torch.cuda.Stream | None = None,
+                 group_levels: np.ndarray | None = None, n_levels0: int = 0):
+        """``cats_np`` holds each cell's batch, or with several covariates
+        (``group_levels``: n_groups x C level ids) each cell's group; theta and
+        N_b then have one entry per level."""
         if not X.is_cuda:
             raise TypeError("HarmonyJob needs X on a HIP device")
         self.lib = _native.load()
@@ -61,6 +65,16 @@ class HarmonyJob:
         self.random_state = int(random_state)
         self.precision = int(precision)
 
+        if group_levels is not None and np.asarray(group_levels).shape[1] >= 2:
+            gl = np.ascontiguousarray(group_levels, dtype=np.int32)
+            self.n_covariates = int(gl.shape[1])
+            self.n_levels = int(np.asarray(N_b_np).size)
+            self.n_levels0 = int(n_levels0)
+        else:
+            gl = None
+            self.n_covariates = 1
+            self.n_levels = self.n_batches
+            self.n_levels0 = self.n_batches
         counts = np.bincount(cats_np, minlength=self.n_batches).astype(np.int32)
         offsets = np.zeros(self.n_batches + 1, dtype=np.int32)
         np.cumsum(counts, out=offsets[1:])
@@ -76,8 +90,12 @@ class HarmonyJob:
             self.N_b = torch.as_tensor(np.asarray(N_b_np, dtype=np_dtype), device=dev)
             self.Pr_b = torch.as_tensor(Pr_b.ravel(), device=dev)
             self.theta = torch.as_tensor(np.ascontiguousarray(theta_np, dtype=np_dtype), device=dev)
-            self.dims = _native.dims(self.n_cells, self.n_pcs, self.n_clusters, self.n_batches,
-                                     self.block_size, self.dsz, self.precision)
+            self.group_levels = (torch.as_tensor(gl, device=dev) if gl is not None else None)
+            self.dims = _native.dims(
+                self.n_cells, self.n_pcs, self.n_clusters, self.n_batches, self.block_size,
+                self.dsz, self.precision, self.n_covariates,
+                self.n_levels if gl is not None else 0, self.n_levels0 if gl is not None else 0,
+                self.group_levels.data_ptr() if gl is not None else None)
             self.ws_bytes = self.lib.hz_workspace_bytes(ctypes.byref(self.dims))
             if self.ws_bytes == 0:
                 raise ValueError("unsupported Harmony shape: " +
@@ -86,9 +104,10 @@ class HarmonyJob:
             self.Z_norm = torch.empty_like(self.X)
             self.Z_hat = torch.empty_like(self.X)
             self.R = torch.empty((self.n_cells, self.n_clusters), dtype=tdt, device=dev)
-            self.E = torch.empty((self.n_batches, self.n_clusters), dtype=tdt, device=dev)
-            self.O = torch.empty((self.n_batches, self.n_clusters), dtype=tdt, device=dev)
-            self.lam = torch.empty((self.n_batches, self.n_clusters), dtype=tdt, device=dev)
+            L = self.n_levels
+            self.E = torch.empty((L, self.n_clusters), dtype=tdt, device=dev)
+            self.O = torch.empty((L, self.n_clusters), dtype=tdt, device=dev)
+            self.lam = torch.empty((L, self.n_clusters), dtype=tdt, device=dev)
             self.obj = torch.zeros(1, dtype=tdt, device=dev)
             self.last_obj = torch.zeros(1, dtype=tdt, device=dev)
         self._n_iter = ctypes.c_int(0)
@@ -101,7 +120,7 @@ class HarmonyJob:
 
     @staticmethod
     def footprint_bytes(n_cells, n_pcs, n_clusters, n_batches, block_size, dsz, precision=0):
-        """Device bytes a job needs (workspace + resident arrays)."""
+        """Device bytes a job with one covariate needs (workspace + resident arrays)."""
         lib = _native.load()
         dm = _native.dims(n_cells, n_pcs, n_clusters, n_batches, block_size, dsz, precision)
         ws = lib.hz_workspace_bytes(ctypes.byref(dm))

@@ -45,6 +45,7 @@ def harmony_integrate(
     verbose: bool = False,
     precision: str = "native",
     init_centroids=None,
+    multi_covariate: bool = False,
 ) -> None:
     """Integrate different experiments using the Harmony algorithm.
 
@@ -54,8 +55,10 @@ def harmony_integrate(
 
     All parameters have the meaning of ``rapids_singlecell.pp.harmony_integrate``.
     Extensions: ``precision`` ("native" or "split_bf16", the latter for
-    float32) and ``init_centroids`` (initial cluster centroids, K x d, used
-    instead of the built-in seeded k-means).
+    float32), ``init_centroids`` (initial cluster centroids, K x d, used
+    instead of the built-in seeded k-means) and ``multi_covariate`` (with a
+    list of keys: one covariate per key, stacked one-hot Phi, instead of the
+    reference's joined categorical).
     """
     from ._harmony import harmonize
 
@@ -144,5 +147,6 @@ def harmony_integrate(
         verbose=verbose,
         init_centroids=init_centroids,
         precision=precision,
+        multi_covariate=multi_covariate,
     )
     adata.obsm[adjusted_basis] = harmony_out.cpu().numpy()

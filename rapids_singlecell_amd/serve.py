@@ -26,6 +26,7 @@ import numpy as np
 import pandas as pd
 import torch
 
+from . import _native
 from .preprocessing._harmony import harmonize
 from .preprocessing._harmony._job import HarmonyJob
 
@@ -146,11 +147,15 @@ class HarmonyServer:
     def _worker(self) -> None:
         with torch.cuda.device(self.device):
             stream = torch.cuda.Stream(device=self.device)
-            while True:
-                job = self._work.get()
-                if job is None:
-                    return
-                self._run(job, stream)
+            try:
+                while True:
+                    job = self._work.get()
+                    if job is None:
+                        return
+                    self._run(job, stream)
+            finally:
+                # the library's per-thread side stream and pinned flags
+                _native.release_thread()
 
     def _run(self, job, stream) -> None:
         fut: Future = job["future"]

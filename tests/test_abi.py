@@ -87,3 +87,30 @@ def test_tuning_roundtrip(lib):
     finally:
         _native.set_tuning(**before)
     assert _native.get_tuning() == before
+
+
+def test_batched_entry_point_validates_on_the_host(lib):
+    """hz_clustering_loop_batched refuses an empty batch, unequal max_iter and
+    jobs of different kernel shapes before any launch (the workspaces are
+    only carved, never touched, so aligned placeholder addresses suffice)."""
+    from rapids_singlecell_amd.preprocessing._harmony._batch import HzJob
+
+    assert lib.hz_batched_workspace_bytes(0, 10) == 0
+    assert lib.hz_batched_workspace_bytes(4, 20) > lib.hz_batched_workspace_bytes(2, 20)
+    assert lib.hz_clustering_loop_batched(None, 0, None, 0, None) == -1
+    arr = (HzJob * 2)()
+    for i, d in enumerate((30, 100)):
+        dm = _dims(5000, d, 40, 3, 250)
+        arr[i].dims = dm
+        arr[i].max_iter = 4
+        arr[i].ws = 1 << 40
+        arr[i].ws_bytes = lib.hz_workspace_bytes(ctypes.byref(dm))
+        arr[i].Z_norm = 1 << 41
+        arr[i].R = 1 << 42
+    p = ctypes.cast(arr, ctypes.c_void_p)
+    arr[1].max_iter = 5
+    assert lib.hz_clustering_loop_batched(p, 2, None, 0, None) == -1
+    assert b"same max_iter" in lib.hz_last_error()
+    arr[1].max_iter = 4
+    assert lib.hz_clustering_loop_batched(p, 2, None, 0, None) == -4
+    assert b"kernel shape" in lib.hz_last_error()

@@ -8,8 +8,15 @@ counts, so that the oracle finishes in well under a minute).
 
 Bounds (relative Frobenius error of Z_corr):
   FP64                       <= 1e-8 of the FP64 oracle, identical iteration counts
-  plain FP32 ("native")      <= 1e-4 of the oracle run in FP64 and in FP32
+  plain FP32 ("native")      <= 1e-4 of the FP64 oracle; and within 1e-4 plus the
+                             oracle's own FP32 rounding (its FP32 run against its
+                             FP64 run) of the oracle run in FP32
   split-BF16                 <= 1e-3 of the FP64 oracle
+The oracle's FP32 run accumulates O, E and the centroid sums in float32 over
+all cells, so at 200k cells it sits about 1.4e-4 from its own FP64 run; the
+GPU path sums O/E in FP64 fixed point and lands much closer to the FP64
+result.  Its distance to the FP32 restatement is therefore bounded by that
+rounding, not by 1e-4 alone.
 Every test prints the error it observed.  Initial centroids are injected
 (K random rows of Z_norm): the reference's cuML k-means cannot run here.
 """
@@ -108,11 +115,12 @@ def test_c1_fp32_fixed_iters(device, c1):
     Z64, _ = c1.oracle(np.float64, **C1_FIXED)
     Z32, _ = c1.oracle(np.float32, **C1_FIXED)
     Zg, rg = c1.gpu(np.float32, **C1_FIXED)
-    e64, e32 = rel(Zg, Z64), rel(Zg, Z32)
+    e64, e32, eo = rel(Zg, Z64), rel(Zg, Z32), rel(Z32, Z64)
     print(f"\nC1 fp32 10x20 iters: rel err Z_corr vs FP64 oracle = {e64:.3e}, "
-          f"vs FP32 oracle = {e32:.3e}", flush=True)
+          f"vs FP32 oracle = {e32:.3e} (oracle FP32 vs FP64: {eo:.3e})", flush=True)
     assert rg["clustering_iters"] == [20] * 10
-    assert e64 < 1e-4 and e32 < 1e-4
+    assert e64 < 1e-4
+    assert e32 < 1e-4 + eo
 
 
 def test_c1_split_bf16_fixed_iters(device, c1):
@@ -136,11 +144,12 @@ def test_c2_fp32_fixed_iters(device, c2):
     Z64, r64 = c2.oracle(np.float64, **C2_FIXED)
     Z32, _ = c2.oracle(np.float32, **C2_FIXED)
     Zg, rg = c2.gpu(np.float32, **C2_FIXED)
-    e64, e32 = rel(Zg, Z64), rel(Zg, Z32)
+    e64, e32, eo = rel(Zg, Z64), rel(Zg, Z32), rel(Z32, Z64)
     eR = rel(rg["R"].cpu().numpy(), r64["R"])
     print(f"\nC2 fp32 2x20 iters: rel err Z_corr vs FP64 oracle = {e64:.3e}, vs FP32 oracle = "
-          f"{e32:.3e}; R vs FP64 oracle = {eR:.3e}", flush=True)
-    assert e64 < 1e-4 and e32 < 1e-4
+          f"{e32:.3e} (oracle FP32 vs FP64: {eo:.3e}); R vs FP64 oracle = {eR:.3e}", flush=True)
+    assert e64 < 1e-4 and eR < 1e-3
+    assert e32 < 1e-4 + eo
 
 
 def test_c2_split_bf16_fixed_iters(device, c2):

@@ -320,13 +320,16 @@ def test_harmonize_tau_matches_oracle(device):
     """tau > 0 discounts theta per batch as theta * (1 - exp(-N_b/(K tau))**2)
     (_harmony/__init__.py:172-175), in the product path as in the oracle."""
     pb = Problem(n=3000, d=20, B=3, K=15, dtype=np.float64)
-    kw = dict(n_clusters=15, max_iter_harmony=3, max_iter_clustering=8, random_state=11, tau=20)
+    # N_b / (K tau) ~ 0.07: the discount keeps about an eighth of theta; a
+    # wider kernel (sigma 0.5) lets the penalty move R, so the discount shows
+    kw = dict(n_clusters=15, max_iter_harmony=3, max_iter_clustering=8, random_state=11, tau=1000,
+              sigma=0.5)
     rec, rec_r = {}, {}
     Zg = harmonize(_t(pb.X, device), pb.obs, "batch", init_centroids=pb.Y0, record=rec, **kw)
     Zr = ho.harmonize(pb.X, pb.codes, pb.B, Y0=pb.Y0, record=rec_r, **kw)
     assert rec["clustering_iters"] == rec_r["clustering_iters"]
     e = rel(Zg.cpu().numpy(), Zr)
-    print(f"\nharmonize tau=20 fp64: rel err vs oracle = {e:.3e}", flush=True)
+    print(f"\nharmonize tau=1000 fp64: rel err vs oracle = {e:.3e}", flush=True)
     assert e < 1e-8
     # the discount changed the result (the test would pass vacuously otherwise)
     Zr0 = ho.harmonize(pb.X, pb.codes, pb.B, Y0=pb.Y0, **dict(kw, tau=0))

@@ -565,3 +565,104 @@ def test_harmonize_convergence_uses_last_two_objectives():
         assert objs[i - 1] - objs[i] >= 1e-2 * abs(objs[i - 1])
     if n_h < 10:
         assert objs[-2] - objs[-1] < 1e-2 * abs(objs[-2])
+
+
+# --------------------------------------------------------------------------
+# Several covariates (stacked one-hot Phi, build extension).  Pinned by two
+# reductions to the one-covariate oracle (itself pinned above) and by the
+# invariants of stacked one-hot bookkeeping.
+# --------------------------------------------------------------------------
+
+
+def _multi_problem(n=600, d=6, B=3, D=4, seed=11):
+    rng = np.random.default_rng(seed)
+    centers = rng.standard_normal((5, d)) * 3
+    b = rng.integers(0, B, n)
+    dn = rng.integers(0, D, n)
+    X = (centers[rng.integers(0, 5, n)] + rng.standard_normal((n, d))
+         + rng.standard_normal((B, d))[b] + 0.5 * rng.standard_normal((D, d))[dn])
+    Z = ho.l2_normalize(X)
+    Y0 = Z[rng.choice(n, 6, replace=False)]
+    return X, b.astype(np.int32), dn.astype(np.int32), Y0
+
+
+MULTI_KW = dict(n_clusters=6, max_iter_harmony=3, max_iter_clustering=6, tol_harmony=-np.inf,
+                tol_clustering=-np.inf, random_state=5, dynamic_lambda=False)
+
+
+@pytest.mark.parametrize("stabilized", [False, True])
+def test_multi_one_covariate_is_the_reference_algorithm(stabilized):
+    X, b, _, Y0 = _multi_problem()
+    r1, r2 = {}, {}
+    Z1 = ho.harmonize(X, b, 3, Y0=Y0, stabilized_penalty=stabilized, record=r1, **MULTI_KW)
+    Z2 = ho.harmonize_multi(X, [b], [3], Y0=Y0, stabilized_penalty=stabilized, record=r2,
+                            **MULTI_KW)
+    np.testing.assert_allclose(r2["objectives"], r1["objectives"], rtol=1e-12)
+    np.testing.assert_allclose(r2["R"], r1["R"], rtol=1e-9, atol=1e-13)
+    assert np.linalg.norm(Z2 - Z1) / np.linalg.norm(Z1) < 1e-10
+
+
+@pytest.mark.parametrize("stabilized", [False, True])
+def test_multi_duplicated_covariate_doubles_theta_and_halves_lambda(stabilized):
+    """A covariate given twice: its two levels per cell multiply the same
+    penalty (theta -> 2 theta) and the ridge solution splits the batch term
+    evenly over the two copies (lambda -> lambda / 2)."""
+    X, b, _, Y0 = _multi_problem(seed=3)
+    r1, r2 = {}, {}
+    kw = dict(MULTI_KW)
+    Z1 = ho.harmonize(X, b, 3, Y0=Y0, theta=3.0, ridge_lambda=0.5, stabilized_penalty=stabilized,
+                      record=r1, **kw)
+    Z2 = ho.harmonize_multi(X, [b, b], [3, 3], Y0=Y0, theta=1.5, ridge_lambda=1.0,
+                            stabilized_penalty=stabilized, record=r2, **kw)
+    np.testing.assert_allclose(r2["objectives"], r1["objectives"], rtol=1e-10)
+    np.testing.assert_allclose(r2["R"], r1["R"], rtol=1e-8, atol=1e-12)
+    assert np.linalg.norm(Z2 - Z1) / np.linalg.norm(Z1) < 1e-9
+
+
+def test_multi_bookkeeping_invariants():
+    X, b, dn, Y0 = _multi_problem(seed=9)
+    rec = {}
+    ho.harmonize_multi(X, [b, dn], [3, 4], Y0=Y0, record=rec, **MULTI_KW)
+    R, O, E = rec["R"], rec["O"], rec["E"]
+    colsum = R.sum(axis=0)
+    # every covariate's levels partition the cells
+    np.testing.assert_allclose(O[:3].sum(axis=0), colsum, rtol=1e-10)
+    np.testing.assert_allclose(O[3:].sum(axis=0), colsum, rtol=1e-10)
+    np.testing.assert_allclose(O[:3], ho._scatter_rows(b, R, 3), rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(O[3:], ho._scatter_rows(dn, R, 4), rtol=1e-9, atol=1e-12)
+    nb = np.concatenate([np.bincount(b, minlength=3), np.bincount(dn, minlength=4)])
+    np.testing.assert_allclose(E, (nb / len(b))[:, None] * colsum[None, :], rtol=1e-9)
+    np.testing.assert_allclose(R.sum(axis=1), 1.0, rtol=1e-12)
+
+
+def test_multi_penalty_is_product_over_levels_brute_force():
+    """One block update on a 9-cell problem, penalties multiplied cell by cell
+    in plain Python."""
+    rng = np.random.default_rng(2)
+    n, K = 9, 3
+    lev = np.array([[rng.integers(0, 2), 2 + rng.integers(0, 3)] for _ in range(n)], np.int32)
+    pen = rng.uniform(0.2, 2.0, size=(5, K))
+    got = ho.multi_penalty_rows(pen, lev)
+    for i in range(n):
+        for k in range(K):
+            assert got[i, k] == pytest.approx(pen[lev[i, 0], k] * pen[lev[i, 1], k], rel=1e-15)
+
+
+def test_multi_correction_solves_the_normal_equations():
+    X, b, dn, _ = _multi_problem(n=300, seed=4)
+    rng = np.random.default_rng(0)
+    R = ho.l1_normalize(rng.uniform(0.01, 1.0, size=(300, 4)))
+    lev, _ = ho.multi_levels([b, dn], [3, 4])
+    lam = np.full((7, 4), 0.7)
+    Z, W = ho.multi_correction(X, R, lam, lev, 7)
+    P = np.zeros((300, 8))
+    P[:, 0] = 1
+    P[np.arange(300), 1 + lev[:, 0]] = 1
+    P[np.arange(300), 1 + lev[:, 1]] = 1
+    for k in range(4):
+        A = (P * R[:, k:k + 1]).T @ P + np.diag(np.concatenate([[0.0], lam[:, k]]))
+        rhs = (P * R[:, k:k + 1]).T @ X
+        full = np.linalg.lstsq(A, rhs, rcond=None)[0]
+        np.testing.assert_allclose(W[k, 1:], full[1:], rtol=1e-8, atol=1e-10)
+    expect = X - sum(R[:, k:k + 1] * (P[:, 1:] @ W[k, 1:]) for k in range(4))
+    np.testing.assert_allclose(Z, expect, rtol=1e-10, atol=1e-12)
