@@ -141,6 +141,17 @@ size_t hz_block_order_workspace_bytes(int n_cells);
 int hz_block_order(int n_cells, uint32_t seed, int32_t* order, void* ws, size_t ws_bytes,
                    void* stream);
 
+/* hz_block_order without a workspace argument and without any allocation:
+ * the scratch is a static device array of 32 MiB per GPU (the library's own,
+ * held until the process ends), so n_cells is limited to what
+ * hz_block_order_workspace_bytes fits in 32 MiB (about 1.6 million cells);
+ * larger inputs return HZ_ERR_WORKSPACE before any launch.  Every call on one
+ * GPU shares that scratch: calls that overlap on one GPU (other threads or
+ * streams) race, so the caller serialises them.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). */
+int hz_block_order_noalloc(int n_cells, uint32_t seed, int32_t* order, void* stream);
+
 /* Block membership of n_iter consecutive clustering iterations (seeds seed ..
  * seed+n_iter-1), built by the same kernels and in the same batched launches
  * as hz_clustering_loop builds them: block_of_cell[y * n_cells + i] =

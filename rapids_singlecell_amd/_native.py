@@ -63,6 +63,7 @@ _SIGNATURES = {
     "hz_l2_normalize": (_i, [_i, _vp, _vp, _ll, _i, _vp]),
     "hz_block_order_workspace_bytes": (_sz, [_i]),
     "hz_block_order": (_i, [_i, _u32, _vp, _vp, _sz, _vp]),
+    "hz_block_order_noalloc": (_i, [_i, _u32, _vp, _vp]),
     "hz_block_membership_workspace_bytes": (_sz, [_i, _i, _i, _i]),
     "hz_block_membership": (_i, [_i, _i, _i, _vp, _u32, _i, _vp, _vp, _vp, _sz, _vp]),
     "hz_init_clusters": (

@@ -1662,6 +1662,30 @@ static int kmeans_impl(const T* Z, long long n, int d, int k, uint32_t seed, int
   return HZ_OK;
 }
 
+// ---------------------------------------------------------------- no-alloc scratch
+
+// Scratch of hz_block_order_noalloc: a file-scope device array, one copy per
+// GPU, held until the process ends.  Host code never uses the array's name as
+// an address: the per-device address comes from hipGetSymbolAddress, taken
+// once per device with that device current.
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+static void* g_scratch_ptr[64];
+static std::mutex g_scratch_mu;
+
+static int scratch_ptr(void** out) {
+  const int dev = current_device_index();
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    HZ_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    HZ_RET_IF(!ptr, HZ_ERR_HIP, "hipGetSymbolAddress returned NULL for the scratch array");
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return HZ_OK;
+}
+
 }  // namespace hz
 
 using namespace hz;
@@ -1754,6 +1778,19 @@ int hz_block_order(int n_cells, uint32_t seed, int32_t* order, void* ws, size_t 
   carve_perm(c, p, b, t);
   b.perm = order;
   return launch_key_order(p, seed, b, nullptr, s);
+}
+
+int hz_block_order_noalloc(int n_cells, uint32_t seed, int32_t* order, void* stream) {
+  HZ_RET_IF(n_cells < 1, HZ_ERR_ARG, "n_cells must be >= 1");
+  HZ_RET_IF(!order, HZ_ERR_ARG, "order is NULL");
+  const size_t need = hz_block_order_workspace_bytes(n_cells);
+  HZ_RET_IF(need > kNoAllocScratchBytes, HZ_ERR_WORKSPACE,
+            "hz_block_order_noalloc: n_cells=%d needs %zu bytes of scratch, the library's "
+            "static scratch holds %zu (use hz_block_order with a workspace)",
+            n_cells, need, kNoAllocScratchBytes);
+  void* ws = nullptr;
+  HZ_TRY(scratch_ptr(&ws));
+  return hz_block_order(n_cells, seed, order, ws, kNoAllocScratchBytes, stream);
 }
 
 // Block membership exactly as the clustering loop builds it (same dims, same
