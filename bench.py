@@ -298,55 +298,231 @@ def run_served(args, duration: float) -> dict:
 
 
 def batched(args) -> int:
-    """Batched form: many independent Harmony instances submitted at once.
+    """Batched form: many independent Harmony instances advanced together.
 
     Per instance: N = floor(exp(U(ln 2000, ln 100000))), d in {30, 50, 100},
     B uniform in [2, 32], K = max(2, floor(min(100, N/30))), 5-40-type
     mixtures with batch shifts, 10 Harmony x 20 clustering iterations (early
-    stop off).  All instances are queued at t=0 on a HarmonyServer; the value
-    is total cells x Harmony iterations over the wall time of the whole set.
+    stop off), initial centroids = K random rows.  ``harmonize_many`` runs the
+    instances of one kernel shape through one multi-job clustering call per
+    Harmony iteration.  The value is the total cells x Harmony iterations over
+    the wall time of the whole set (inputs already on the device, from the
+    first initialisation to the last correction).
     """
     import torch
 
     from rapids_singlecell_amd import _native
     from rapids_singlecell_amd.datasets import synthetic_pcs
-    from rapids_singlecell_amd.serve import HarmonyServer, latency_summary
+    from rapids_singlecell_amd.preprocessing._harmony._batch import harmonize_many
 
     _native.load()
     rng = np.random.default_rng(args.batched_seed)
-    jobs = []
+    probs = []
     for i in range(args.batched_n):
         n = int(np.exp(rng.uniform(np.log(2_000), np.log(100_000))))
         d = int(rng.choice([30, 50, 100]))
         B = int(rng.integers(2, 33))
+        K = max(2, int(min(100, n / 30)))
         X, obs = synthetic_pcs(n, d, int(rng.integers(5, 41)), B, seed=5000 + i)
-        codes = np.asarray(obs["batch"].cat.codes.values, dtype=np.int32)
-        jobs.append((X, codes, len(obs["batch"].cat.categories)))
-    kw = dict(max_iter_harmony=10, max_iter_clustering=20, tol_harmony=float("-inf"),
-              tol_clustering=float("-inf"), precision=args.precision)
-    with HarmonyServer(slots=args.served_slots) as srv:
-        srv.submit(*jobs[0], **dict(kw, max_iter_harmony=1)).result()  # warm-up
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        futs = [srv.submit(X, c, B, **kw) for X, c, B in jobs]
-        res = [f.result() for f in futs]
-        wall = time.perf_counter() - t0
-    cells_iters = sum(info["n_cells"] * info["harmony_iters"] for _, info in res)
+        Y0 = X[rng.choice(n, K, replace=False)]
+        probs.append(dict(X=torch.as_tensor(X, device="cuda"), obs=obs, key="batch",
+                          n_clusters=K, init_centroids=Y0, max_iter_harmony=10,
+                          max_iter_clustering=20, tol_harmony=float("-inf"),
+                          tol_clustering=float("-inf"), precision=args.precision,
+                          random_state=i))
+    print(f"[bench] batched: {len(probs)} instances ready", file=sys.stderr, flush=True)
+    harmonize_many(probs[:2])  # warm-up (library load, kernel first launches)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out_z = harmonize_many(probs)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    assert all(torch.isfinite(z).all().item() for z in out_z)
+    cells_iters = sum(p["X"].shape[0] * 10 for p in probs)
     out = {
         "metric": METRIC + " (batched form)",
         "value": round(cells_iters / wall, 1),
         "unit": "cells·iters/s",
         "n_gpus": 1,
         "higher_is_better": True,
-        "instances": len(res),
-        "total_cells": int(sum(j[0].shape[0] for j in jobs)),
+        "instances": len(probs),
+        "total_cells": int(sum(p["X"].shape[0] for p in probs)),
         "wall_s": round(wall, 3),
-        "latency": latency_summary([info["latency_s"] for _, info in res]),
-        "slots": args.served_slots,
         "precision": args.precision,
         "data": "synthetic Gaussian-mixture PCA embeddings, one per instance",
     }
     print(json.dumps(out), flush=True)
+    return 0
+
+
+def batched_leg(X, codes, B, K, precision: str, copies: int, warmup: int, steps: int) -> dict:
+    """``copies`` instances of the workload (same data, different random
+    states, so different block orders) stepped together: one multi-job
+    clustering call per Harmony iteration, then every correction."""
+    import torch
+
+    from rapids_singlecell_amd.preprocessing._harmony._batch import step_many
+    from rapids_singlecell_amd.preprocessing._harmony._job import HarmonyJob
+
+    prec = {"native": 0, "split_bf16": 1}[precision]
+    stream = torch.cuda.current_stream()
+    dev = torch.device("cuda", 0)
+    Xd = torch.as_tensor(X, device=dev)
+    N_b = np.bincount(codes, minlength=B).astype(np.float32)
+    jobs = []
+    for c in range(copies):
+        job = HarmonyJob(Xd, codes, B, n_clusters=K, theta_np=np.full(B, 2.0, np.float32),
+                         N_b_np=N_b, sigma=0.1, block_proportion=0.05, max_iter_clustering=20,
+                         tol_clustering=float("-inf"), ridge_lambda=1.0,
+                         stabilized_penalty=False, dynamic_lambda=False,
+                         correction_method="batched", random_state=c, precision=prec,
+                         stream=stream)
+        job.init()
+        jobs.append(job)
+    torch.cuda.synchronize()
+    it = 0
+    for _ in range(warmup):
+        step_many(jobs, it)
+        it += 1
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step_many(jobs, it)
+        it += 1
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    n = X.shape[0]
+    leg = {"precision": precision, "instances": copies,
+           "value": round(copies * n * steps / dt, 1),
+           "ms_per_step": round(1e3 * dt / steps, 3), "steps": steps}
+    print(f"[bench] batched {precision} x{copies}: {steps} steps in {dt:.3f} s -> "
+          f"{leg['value']:.4g} cells·iters/s", file=sys.stderr, flush=True)
+    del jobs
+    torch.cuda.empty_cache()
+    return leg
+
+
+def multi_bench(args) -> int:
+    """Several covariates (the precision-form configuration): 1.3M cells x 50
+    PCs, 50-type mixture, two independent uniform covariates (16 batches and
+    64 donors, 80 levels of stacked one-hot Phi), per-level shifts
+    ~N(0, 0.5^2), K = 100, harmony1 lambda = 1.  Times Harmony iterations of
+    the multi-covariate path in FP32, split-BF16 and FP64 (inputs in HBM) and
+    reports the error of the FP32 paths against the FP64 path after two
+    Harmony iterations from the same initial centroids."""
+    import torch
+
+    from rapids_singlecell_amd import _native
+    from rapids_singlecell_amd.preprocessing._harmony._helper import _get_covariates
+    from rapids_singlecell_amd.preprocessing._harmony._job import HarmonyJob
+
+    _native.load()
+    n, d, K = args.multi_cells, 50, 100
+    rng = np.random.default_rng(0)
+    centers = rng.normal(0, 3.0, size=(50, d))
+    b = rng.integers(0, 16, n)
+    dn = rng.integers(0, 64, n)
+    X = (centers[rng.integers(0, 50, n)] + rng.normal(size=(n, d))
+         + rng.normal(0, 0.5, size=(16, d))[b] + rng.normal(0, 0.5, size=(64, d))[dn])
+    X = X.astype(np.float32)
+    import pandas as pd
+
+    obs = pd.DataFrame({"batch": pd.Categorical(b), "donor": pd.Categorical(dn)})
+    _, n_levels, group_levels, groups = _get_covariates(obs, ["batch", "donor"])
+    N_b = np.bincount(group_levels[groups].ravel(), minlength=sum(n_levels))
+    Y0 = initial_centroids(X, K)
+
+    def make(Xh, prec):
+        dt = Xh.dtype
+        job = HarmonyJob(torch.as_tensor(Xh, device="cuda"), groups, len(group_levels),
+                         n_clusters=K, theta_np=np.full(sum(n_levels), 2.0, dt),
+                         N_b_np=N_b.astype(dt), sigma=0.1, block_proportion=0.05,
+                         max_iter_clustering=20, tol_clustering=float("-inf"), ridge_lambda=1.0,
+                         stabilized_penalty=False, dynamic_lambda=False, random_state=0,
+                         precision=prec, group_levels=group_levels, n_levels0=n_levels[0])
+        job.init(torch.as_tensor(Y0.astype(dt)))
+        return job
+
+    legs = {}
+    final = {}
+    for name, Xh, prec in (("fp32", X, 0), ("split_bf16", X, 1), ("fp64", X.astype(np.float64), 0)):
+        job = make(Xh, prec)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        job.step(0)
+        job.step(1)
+        torch.cuda.synchronize()
+        dt_ = (time.perf_counter() - t0) / 2
+        legs[name] = {"value": round(n / dt_, 1), "ms_per_step": round(1e3 * dt_, 2)}
+        final[name] = job.Z_hat.double()
+        print(f"[bench] multi {name}: {1e3 * dt_:.1f} ms per Harmony iteration", file=sys.stderr,
+              flush=True)
+        del job
+        torch.cuda.empty_cache()
+    ref = final["fp64"]
+    errs = {k: float(torch.linalg.norm(final[k] - ref) / torch.linalg.norm(ref))
+            for k in ("fp32", "split_bf16")}
+    print(json.dumps({
+        "metric": "Harmony cells·iters/s, several covariates (stacked one-hot Phi)",
+        "unit": "cells·iters/s", "higher_is_better": True,
+        "config": {"workload": f"N={n} d={d} K={K} covariates 16 batches + 64 donors "
+                               f"({len(group_levels)} groups, 80 levels), harmony1 lambda=1, "
+                               "20 clustering iters per Harmony iter"},
+        "legs": legs, "rel_err_Z_corr_vs_fp64_after_2_iters": errs,
+    }), flush=True)
+    return 0
+
+
+def knn_bench(args) -> int:
+    """Brute-force kNN (the neighbour search on the integrated embedding):
+    every row of an N x 50 embedding against all rows, k = 15, euclidean.
+    Timed with HIP events around ``hz_knn`` (inputs on the device); the
+    roofline is the matrix-core bound, 2 N^2 d useful flops against the FP32
+    MFMA peak."""
+    import ctypes
+
+    import torch
+
+    from rapids_singlecell_amd import _native
+    from rapids_singlecell_amd.datasets import synthetic_pcs
+
+    lib = _native.load()
+    out = []
+    for n in args.knn_sizes:
+        X, _ = synthetic_pcs(n, 50, 30, 8, seed=1)
+        Xd = torch.as_tensor(X, device="cuda")
+        k = 15
+        idx = torch.empty((n, k), dtype=torch.int32, device="cuda")
+        dist = torch.empty((n, k), dtype=torch.float32, device="cuda")
+        nb = lib.hz_knn_workspace_bytes(n, 50, 0)
+        ws = torch.empty(max(nb, 256), dtype=torch.uint8, device="cuda")
+        s = torch.cuda.current_stream()
+
+        def run():
+            _native.check(lib.hz_knn(0, ctypes.c_void_p(Xd.data_ptr()), n, 50, k,
+                                     ctypes.c_void_p(idx.data_ptr()),
+                                     ctypes.c_void_p(dist.data_ptr()),
+                                     ctypes.c_void_p(ws.data_ptr()), ws.numel(),
+                                     ctypes.c_void_p(s.cuda_stream)), "hz_knn")
+
+        run()
+        torch.cuda.synchronize()
+        reps = 3 if n <= 300_000 else 1
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        for _ in range(reps):
+            run()
+        e1.record(s)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        tflops = 2.0 * n * n * 50 / (ms * 1e-3) / 1e12
+        out.append({"n_cells": n, "n_pcs": 50, "k": k, "ms": round(ms, 3),
+                    "roofline": {"bound": "mfma", "achieved": round(tflops, 2), "peak": 157.3,
+                                 "unit": "TFLOP/s", "frac": round(tflops / 157.3, 4)}})
+        print(f"[bench] knn n={n}: {ms:.2f} ms, {tflops:.1f} TFLOP/s", file=sys.stderr, flush=True)
+        del Xd, idx, dist, ws
+    print(json.dumps({"metric": "brute-force kNN time (k=15, euclidean, FP32)",
+                      "unit": "ms", "higher_is_better": False, "results": out}), flush=True)
     return 0
 
 
@@ -468,6 +644,11 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-errors", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--fp64-steps", type=int, default=2,
+                    help="timed steps of the FP64 leg of the default run (0 = skip)")
+    ap.add_argument("--batch-copies", type=int, default=2,
+                    help="instances of the workload in the batched leg of the default run "
+                         "(1 = skip)")
     ap.add_argument("--served-leg", type=float, default=12.0,
                     help="seconds of the served leg of the default run (0 = skip)")
     ap.add_argument("--pmc-run", action="store_true",
@@ -486,6 +667,12 @@ def main() -> int:
                     help="batched form: 64 uneven instances queued at once")
     ap.add_argument("--batched-n", type=int, default=64)
     ap.add_argument("--batched-seed", type=int, default=0)
+    ap.add_argument("--multi", action="store_true",
+                    help="several covariates (16 batches + 64 donors) at --multi-cells")
+    ap.add_argument("--multi-cells", type=int, default=1_300_000)
+    ap.add_argument("--knn", action="store_true",
+                    help="time the brute-force kNN of the neighbour graph")
+    ap.add_argument("--knn-sizes", type=int, nargs="+", default=[200_000, 1_300_000])
     ap.add_argument("--tune", action="append", default=[],
                     help="library tuning knob key=value (row_chunk, centroid_wg, no_sweep, sweep_wg)")
     args = ap.parse_args()
@@ -500,6 +687,10 @@ def main() -> int:
         return batched(args)
     if args.end_to_end:
         return end_to_end(args)
+    if args.knn:
+        return knn_bench(args)
+    if args.multi:
+        return multi_bench(args)
 
     import torch
 
@@ -533,6 +724,16 @@ def main() -> int:
     legs = {p: throughput_leg(X, codes, B, K, p, args.warmup, args.steps, roof_on)
             for p in ("native", "split_bf16")}
     head = legs["split_bf16"]
+    # the reference's default dtype is float64: the FP64 path (per-block
+    # launches, FP64 kernels) on the same workload, a few steps
+    fp64 = None
+    if args.fp64_steps > 0:
+        fp64 = throughput_leg(X.astype(np.float64), codes, B, K, "native", 1, args.fp64_steps,
+                              roof_on)
+    batch = None
+    if args.batch_copies > 1:
+        batch = batched_leg(X, codes, B, K, "split_bf16", args.batch_copies, args.warmup,
+                            args.steps)
     errors = None if args.no_errors else error_leg(X, codes, B, K)
     served_out = None
     if args.served_leg > 0:
@@ -544,14 +745,15 @@ def main() -> int:
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         cpu = cpu_baseline(X, codes, B, K)
+    value_leg = batch if batch is not None else head
     out = {
         "metric": METRIC,
-        "value": head["value"],
+        "value": value_leg["value"],
         "unit": "cells·iters/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": head["ms_per_step"],
+        "ms_per_step": value_leg["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -559,12 +761,19 @@ def main() -> int:
         "data": "synthetic: Gaussian-mixture PCA embedding with per-batch shifts (no datasets offline)",
         "config": {
             "workload": workload_text(args.workload),
-            "value_leg": "split_bf16, one job",
+            "value_leg": (f"split_bf16, {batch['instances']} instances of the workload stepped "
+                          "together (one multi-job clustering call per Harmony iteration)"
+                          if batch is not None else "split_bf16, one job"),
             "tuning": _native.get_tuning(),
             "n_cells": n, "n_pcs": d, "n_clusters": K, "n_batches": B,
         },
         "fp32": {k: legs["native"][k] for k in ("value", "ms_per_step")},
-        "split_bf16_vs_fp32": round(head["value"] / legs["native"]["value"], 3),
+        "split_bf16_single": {k: head[k] for k in ("value", "ms_per_step")},
+        "fp64": ({k: fp64[k] for k in ("value", "ms_per_step", "roofline", "kernels")}
+                 if fp64 else None),
+        "split_bf16_batched": batch,
+        "value_vs_fp32": round(value_leg["value"] / legs["native"]["value"], 3),
+        "split_bf16_single_vs_fp32": round(head["value"] / legs["native"]["value"], 3),
         "rel_err_vs_fp64": errors,
         "latency": served_out["latency"] if served_out else None,
         "served": served_out,

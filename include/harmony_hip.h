@@ -286,6 +286,53 @@ size_t hz_kmeans_workspace_bytes(int dtype, long long n_cells, int n_pcs, int k)
 int hz_kmeans(int dtype, const void* Z_norm, long long n_cells, int n_pcs, int k, uint32_t seed,
               int max_iter, void* Y_out, void* ws, size_t ws_bytes, void* stream);
 
+/* Exact k nearest neighbours (brute force) of every row of X (n_rows x
+ * n_cols, float32, device) among all rows of X, nearest first: indices and
+ * distances are n_rows x k (the row itself is among them).  Metrics:
+ * HZ_KNN_EUCLIDEAN, HZ_KNN_SQEUCLIDEAN (expanded form |x|^2 + |y|^2 - 2 x.y,
+ * clamped at 0) and HZ_KNN_COSINE (1 - x.y of unit rows).  k <= 32,
+ * n_cols <= 128; neighbours are ordered by (distance, index).  Replaces
+ * cuML NearestNeighbors(algorithm="brute").kneighbors
+ * (preprocessing/_neighbors/_algorithms/_brute.py:14-38), called by
+ * pp.neighbors (preprocessing/_neighbors/__init__.py:200-207). */
+#define HZ_KNN_EUCLIDEAN 0
+#define HZ_KNN_SQEUCLIDEAN 1
+#define HZ_KNN_COSINE 2
+size_t hz_knn_workspace_bytes(long long n_rows, int n_cols, int metric);
+int hz_knn(int metric, const float* X, long long n_rows, int n_cols, int k, int32_t* indices,
+           float* distances, void* ws, size_t ws_bytes, void* stream);
+
+/* UMAP fuzzy-set membership weights of a kNN graph (n_rows x k, nearest
+ * first, as hz_knn returns it): per row the smooth-kNN normalisation (rho =
+ * nearest non-zero distance, sigma by bisection so that the memberships of
+ * neighbours 1..k-1 sum to log2(k), floored at 1e-3 of the row's mean
+ * distance, or of mean_distance -- the mean of all n_rows x k distances --
+ * when rho = 0), then w = exp(-max(0, d - rho) / sigma), 0 for the row
+ * itself.  sigmas and rhos (n_rows each) are optional outputs.  Replaces the
+ * membership half of cuML's fuzzy_simplicial_set as called by
+ * _get_connectivities_umap (preprocessing/_neighbors/_neighbors.py:134-159);
+ * the symmetrisation w + w^T - w*w^T is done by the caller on the sparse
+ * graph. */
+int hz_umap_weights(const float* distances, const int32_t* indices, long long n_rows, int k,
+                    double mean_distance, float* weights, float* sigmas, float* rhos,
+                    void* stream);
+
+/* Dense PCA with zero centring, the producer of the Harmony input
+ * (obsm["X_pca"]).  hz_pca_covariance: column means (mean, n_cols, FP64) and
+ * the covariance of the centred X (cov, n_cols x n_cols, FP64, divided by
+ * n_rows - 1) of a float32 X (n_rows x n_cols); hz_pca_project: X_pca =
+ * (X - mean) components (n_cols x n_comps float32, n_comps <= 64) into out
+ * (n_rows x n_comps).  The eigendecomposition of cov is the caller's.  Both
+ * take a workspace of hz_pca_workspace_bytes(n_rows, n_cols) bytes.
+ * Replaces the dense zero-centred branch of pp.pca (cuML PCA, svd_solver
+ * "full"/"covariance_eigh"; preprocessing/_pca.py:71-330). */
+size_t hz_pca_workspace_bytes(long long n_rows, int n_cols);
+int hz_pca_covariance(const float* X, long long n_rows, int n_cols, double* mean, double* cov,
+                      void* ws, size_t ws_bytes, void* stream);
+int hz_pca_project(const float* X, long long n_rows, int n_cols, const double* mean,
+                   const float* components, int n_comps, float* out, void* ws, size_t ws_bytes,
+                   void* stream);
+
 #ifdef __cplusplus
 }
 #endif

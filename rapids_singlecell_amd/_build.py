@@ -17,6 +17,8 @@ LIB_PATH = PKG / "_lib" / LIB_NAME
 # per precision, so the units compile in parallel
 UNITS = [
     ("hz_capi.hip", {}),
+    ("hz_knn.hip", {}),
+    ("hz_pca.hip", {}),
     *[(f"hz_k_{k}.hip", {"HZ_UNIT_PREC": p}) for k in ("sweep", "row", "cent") for p in (0, 1)],
 ]
 ARCH = os.environ.get("HZ_OFFLOAD_ARCH", "gfx950")

@@ -90,6 +90,12 @@ _SIGNATURES = {
     "hz_fused_pen_norm": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _d, _i, _i, _vp]),
     "hz_lambda_kb": (_i, [_i, _vp, _vp, _vp, _i, _i, _i, _d, _i, _d, _d, _vp, _vp]),
     "hz_inv_mats": (_i, [_i, _vp, _vp, _vp, _i, _i, _vp]),
+    "hz_knn_workspace_bytes": (_sz, [_ll, _i, _i]),
+    "hz_knn": (_i, [_i, _vp, _ll, _i, _i, _vp, _vp, _vp, _sz, _vp]),
+    "hz_umap_weights": (_i, [_vp, _vp, _ll, _i, _d, _vp, _vp, _vp, _vp]),
+    "hz_pca_workspace_bytes": (_sz, [_ll, _i]),
+    "hz_pca_covariance": (_i, [_vp, _ll, _i, _vp, _vp, _vp, _sz, _vp]),
+    "hz_pca_project": (_i, [_vp, _ll, _i, _vp, _vp, _i, _vp, _vp, _sz, _vp]),
     "hz_kmeans_workspace_bytes": (_sz, [_i, _ll, _i, _i]),
     "hz_kmeans": (_i, [_i, _vp, _ll, _i, _i, _u32, _i, _vp, _vp, _sz, _vp]),
 }

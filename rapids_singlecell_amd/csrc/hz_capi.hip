@@ -433,7 +433,7 @@ static int launch_centroid(const Layout& L, const CentArgs<T>& a, int grid, hipS
 }
 
 static int launch_insum(const Layout& L, const InSumArgs& a, hipStream_t s) {
-  const int grid = (int)std::max<long long>(1, ceil_div(L.pd.maxch, kBlock / kWave));
+  const int grid = (int)std::max<long long>(1, ceil_div(ceil_div(L.n, kInSumSpan), kBlock / kWave));
   switch ((int)ceil_div(L.K, kWave)) {
     case 1: hipLaunchKernelGGL(insum_kernel<1>, dim3(grid), dim3(kBlock), 0, s, a); break;
     case 2: hipLaunchKernelGGL(insum_kernel<2>, dim3(grid), dim3(kBlock), 0, s, a); break;
@@ -938,6 +938,7 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
   ia.R = reinterpret_cast<const float*>(R);
   ia.done = &L.ctrl->done;
   ia.acc_in = L.acc_in;
+  ia.n = L.n;
   ia.nseg = (int)nseg;
   ia.K = L.K;
   ia.scale_o = L.scale_o;
@@ -977,10 +978,7 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
     if (it >= 1) HZ_HIP(hipStreamWaitEvent(s2, ss->freed[(it - 1) & 1], 0));
     if (use_sw) {
       ia.order = t.order;
-      ia.seg_ch = t.seg_ch;
-      ia.ch_start = t.ch_start;
-      ia.ch_len = t.ch_len;
-      ia.ch_seg = t.ch_seg;
+      ia.seg_start = t.seg_start;
       ProfScope ps(HZ_PROF_ORDER, s2);
       HZ_TRY(fill<long long>(L.acc_in, L.nacc * L.K, 0, s2));
       HZ_TRY(launch_insum(L, ia, s2));
@@ -1215,6 +1213,7 @@ static int batched_impl(hz_job* jobs, int n, void* bws, size_t bws_bytes, hipStr
     b.ia.R = R;
     b.ia.done = &L.ctrl->done;
     b.ia.acc_in = L.acc_in;
+    b.ia.n = L.n;
     b.ia.nseg = (int)((long long)L.pd.nblk * L.B);
     b.ia.K = L.K;
     b.ia.scale_o = L.scale_o;
@@ -1273,10 +1272,7 @@ static int batched_impl(hz_job* jobs, int n, void* bws, size_t bws_bytes, hipStr
       Layout& L = b.L;
       const ChunkTab t = tab_of(L.ct, L.pd, it % L.nord);
       b.ia.order = t.order;
-      b.ia.seg_ch = t.seg_ch;
-      b.ia.ch_start = t.ch_start;
-      b.ia.ch_len = t.ch_len;
-      b.ia.ch_seg = t.ch_seg;
+      b.ia.seg_start = t.seg_start;
       ProfScope ps(HZ_PROF_ORDER, s2);
       HZ_TRY(fill<long long>(L.acc_in, L.nacc * L.K, 0, s2));
       HZ_TRY(launch_insum(L, b.ia, s2));

@@ -489,13 +489,20 @@ __global__ __launch_bounds__(kBlock) void yreduce_kernel(YRedArgs<T> a) {
     part[w * d + c] = s[0];
   }
   __syncthreads();
-  if (w == 0)
-    for (int c = lane; c < d; c += kWave)
-      a.Yslice[((long long)sl * a.K + k) * d + c] =
-          part[c] + part[d + c] + part[2 * d + c] + part[3 * d + c];
-  // arrival of this slice of row k: the writes above are made visible to the
-  // device before the count, and the last slice reads the others after it
-  __threadfence();
+  // this slice's sums go out as device-scope atomic exchanges whose old
+  // values are consumed before the arrival count: the count then orders
+  // after them without a device-wide fence (an L2 write-back), and the last
+  // slice reads them back with device-scope loads
+  if (w == 0) {
+    unsigned long long sink = 0;
+    for (int c = lane; c < d; c += kWave) {
+      const double v = part[c] + part[d + c] + part[2 * d + c] + part[3 * d + c];
+      sink += __hip_atomic_exchange(
+          reinterpret_cast<unsigned long long*>(&a.Yslice[((long long)sl * a.K + k) * d + c]),
+          __builtin_bit_cast(unsigned long long, v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    asm volatile("" ::"v"(sink));
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
     const int prev = __hip_atomic_fetch_add(&a.ycnt[k], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -503,14 +510,15 @@ __global__ __launch_bounds__(kBlock) void yreduce_kernel(YRedArgs<T> a) {
   }
   __syncthreads();
   if (s_last) {
-    __threadfence();
     // row sum of squares in T, as l2_row_normalize_kernel
     if (w == 0) {
       T ss = T(0);
       T yv[2] = {T(0), T(0)};
       for (int c = lane, u = 0; c < d; c += kWave, ++u) {
         double v = 0.0;
-        for (int q = 0; q < ns; ++q) v += a.Yslice[((long long)q * a.K + k) * d + c];
+        for (int q = 0; q < ns; ++q)
+          v += __builtin_bit_cast(double, (unsigned long long)load_agent(reinterpret_cast<const long long*>(
+                                              &a.Yslice[((long long)q * a.K + k) * d + c])));
         T t = T(v);
         if (u < 2) yv[u] = t;
         a.Y[(long long)k * d + c] = t;

@@ -44,75 +44,73 @@ constexpr int kSweepRep = 4;
 struct InSumArgs {
   const float* R;
   const int* order;
-  const int* seg_ch;
-  const int* ch_start;
-  const int* ch_len;
-  const int* ch_seg;
-  long long* acc_in;  // nseg x K, zero at launch
-  const int* done;    // convergence flag: nothing to do once set
-  int nseg, K;
+  const int* seg_start;  // nseg + 1 positions (into order) where each segment starts
+  long long* acc_in;     // nseg x K, zero at launch
+  const int* done;       // convergence flag: nothing to do once set
+  int n, nseg, K;
   double scale_o;
 };
+
+constexpr int kInSumSpan = 256;  // positions of the order per wave
 
 // Sums of the rows of R (as left by the previous clustering iteration) over
 // every (block, batch) segment of this iteration's block order: the "remove
 // the block's old contribution" terms of clustering.cu:321-334, for all blocks
-// at once.  One wave per chunk; consecutive waves of one segment are combined
-// in LDS (fixed order) before a fixed-point add.
+// at once.  Each wave walks a fixed span of consecutive positions of the
+// grouped order (segment-major), 16 rows at a time with every row's loads
+// issued together, and adds its per-segment column sums to the fixed-point
+// accumulators once per segment it touches (lanes own columns).
 template <int KW>
 __global__ __launch_bounds__(kBlock) void insum_kernel(InSumArgs a) {
-  constexpr int NW = kBlock / kWave;
-  __shared__ float s_part[NW][kWave * KW];
-  __shared__ int s_seg[NW];
-  const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
   if (*a.done) return;
   const int K = a.K;
-  const int nch = a.seg_ch[a.nseg];
-  const int c = blockIdx.x * NW + wv;
-  float ks[KW];
+  const long long wv = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int p0 = (int)(wv * kInSumSpan);
+  if (p0 >= a.n) return;
+  const int p1 = min(a.n, p0 + kInSumSpan);
+  // segment of the first position: the last s with seg_start[s] <= p0
+  int lo = 0, hi = a.nseg;  // seg_start[0] = 0 <= p0 < seg_start[nseg] = n
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (a.seg_start[mid] <= p0)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  int seg = lo;
+  int p = p0;
+  while (p < p1) {
+    const int pe = min(p1, a.seg_start[seg + 1]);
+    float ks[KW];
 #pragma unroll
-  for (int w = 0; w < KW; ++w) ks[w] = 0.f;
-  int seg = -1;
-  if (c < nch) {
-    seg = a.ch_seg[c];
-    const int p0 = a.ch_start[c];
-    const int len = a.ch_len[c];
-    const int cell = (lane < len) ? a.order[p0 + lane] : -1;
-    for (int r0 = 0; r0 < len; r0 += 16) {
+    for (int w = 0; w < KW; ++w) ks[w] = 0.f;
+    for (int r0 = p; r0 < pe; r0 += 16) {
+      const int mine = r0 + (lane & 15);
+      const int cell = a.order[min(mine, pe - 1)];  // always a valid position
+      float x[16][KW];
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
-        const int rr = r0 + r;
-        const int rc = __shfl(cell, rr & 63, kWave);
-        const bool ok = rr < len;
-        const unsigned base = (unsigned)(ok ? rc : 0) * (unsigned)K;
+        const int rc = __shfl(cell, r, kWave);
+        const unsigned base = (unsigned)rc * (unsigned)K;
 #pragma unroll
-        for (int w = 0; w < KW; ++w) {
-          const int k = lane + kWave * w;
-          const float x = a.R[base + (unsigned)min(k, K - 1)];
-          // multiplicative mask: the (always valid) loads stay unconditional
-          ks[w] += x * ((ok && k < K) ? 1.f : 0.f);
-        }
+        for (int w = 0; w < KW; ++w) x[r][w] = a.R[base + (unsigned)min(lane + kWave * w, K - 1)];
+      }
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        // multiplicative mask: the (always valid) loads stay unconditional
+        const float m = (r0 + r < pe) ? 1.f : 0.f;
+#pragma unroll
+        for (int w = 0; w < KW; ++w) ks[w] = __builtin_fmaf(x[r][w], m, ks[w]);
       }
     }
-  }
 #pragma unroll
-  for (int w = 0; w < KW; ++w) s_part[wv][lane + kWave * w] = ks[w];
-  if (lane == 0) s_seg[wv] = seg;
-  __syncthreads();
-  for (int k = tid; k < K; k += kBlock) {
-    int w = 0;
-    while (w < NW) {
-      const int sg = s_seg[w];
-      if (sg < 0) {
-        ++w;
-        continue;
-      }
-      float x = 0.f;
-      int w2 = w;
-      while (w2 < NW && s_seg[w2] == sg) x += s_part[w2++][k];
-      fixed_add(&a.acc_in[(long long)sg * K + k], (double)x, a.scale_o);
-      w = w2;
+    for (int w = 0; w < KW; ++w) {
+      const int k = lane + kWave * w;
+      if (k < K) fixed_add(&a.acc_in[(long long)seg * K + k], (double)ks[w], a.scale_o);
     }
+    p = pe;
+    ++seg;
   }
 }
 
@@ -244,7 +242,8 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& a, const int wg, con
   float* s_E = s_O + BK;
   float* s_lpen = s_E + BK;  // log of the penalty
   double* s_do = reinterpret_cast<double*>(s_lpen + BK);  // new-row deltas (update)
-  float* s_dO = reinterpret_cast<float*>(s_do);          // per-block sums (rows), aliased
+  // per-block sums of the new rows (fixed point, scale_pk), aliased
+  long long* s_dO = reinterpret_cast<long long*>(s_do);
   double* s_di = s_do + BK;                              // old-row deltas of the next block
   double* s_co = s_di + BK;
   double* s_ci = s_co + K;
@@ -555,14 +554,16 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& a, const int wg, con
     __syncthreads();
   };
 
-  // rows of block j for this workgroup (new R; per-block sums into s_dO).
+  // rows of block j for this workgroup (new R; per-block sums into s_dO,
+  // each chunk's sum rounded to fixed point on its own, so the workgroup's
+  // totals do not depend on which chunks it was given).
   // Round 0 uses the prefetched tile; the Z rows of round r+1 are loaded
   // while round r is normalised.
   auto rows_block = [&](int j) __attribute__((always_inline)) {
     int lo, hi;
     wg_range(j, lo, hi);
     const int sb = j * B;
-    for (int q = tid; q < BK; q += kBlock) s_dO[q] = 0.f;
+    for (int q = tid; q < BK; q += kBlock) s_dO[q] = 0;
     float zc[NA];
     int cseg = -1, clen = 0, ccell = -1;
     auto round_chunk = [&](int r, int& seg, int& len, int& cell) __attribute__((always_inline)) {
@@ -635,7 +636,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& a, const int wg, con
 #pragma unroll
         for (int w = 0; w < NW; ++w) {
           const int sg = s_seg[w];
-          if (sg >= 0) s_dO[(sg - sb) * K + k] += s_part[w][k];
+          if (sg >= 0) s_dO[(sg - sb) * K + k] += llrint((double)s_part[w][k] * a.scale_pk);
         }
       }
       __syncthreads();
@@ -653,7 +654,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& a, const int wg, con
     for (int q = tid; q < nq; q += kBlock) {
       const int p = bl * K + q;
       const unsigned long long v =
-          (1ull << kCountShift) + (unsigned long long)llrint((double)s_dO[p] * a.scale_pk);
+          (1ull << kCountShift) + (unsigned long long)s_dO[p];
       __hip_atomic_fetch_add(reinterpret_cast<unsigned long long*>(&out_rep[(long long)sb * K + p]),
                              v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }

@@ -133,6 +133,40 @@ def clustering_batched(jobs: list[HarmonyJob], harmony_iter: int) -> list[int]:
     return [arr[i].n_iter for i in range(n)]
 
 
+def clustering_step(jobs: list[HarmonyJob], harmony_iter: int,
+                    max_group: int | None = None) -> list[int]:
+    """Clustering step of Harmony iteration ``harmony_iter`` for every job:
+    jobs of one sweep class together (at most ``max_group`` per call), the
+    others alone.  Returns each job's clustering iterations."""
+    out = [0] * len(jobs)
+    groups: dict = {}
+    for i, job in enumerate(jobs):
+        c = sweep_class(job)
+        if c is None:
+            out[i] = job.clustering(harmony_iter)
+        else:
+            groups.setdefault(c, []).append(i)
+    for members in groups.values():
+        step = max_group or len(members)
+        for g0 in range(0, len(members), step):
+            part = members[g0:g0 + step]
+            if len(part) == 1:
+                out[part[0]] = jobs[part[0]].clustering(harmony_iter)
+                continue
+            for i, nc in zip(part, clustering_batched([jobs[i] for i in part], harmony_iter)):
+                out[i] = nc
+    return out
+
+
+def step_many(jobs: list[HarmonyJob], harmony_iter: int, max_group: int | None = None) -> list[int]:
+    """One Harmony iteration (clustering, then each job's correction) of
+    several jobs that share a stream."""
+    out = clustering_step(jobs, harmony_iter, max_group)
+    for job in jobs:
+        job.correction()
+    return out
+
+
 def harmonize_many(problems: list[dict], *, max_group: int | None = None,
                    stream: torch.cuda.Stream | None = None, record: list | None = None):
     """Run several Harmony problems together.
@@ -201,22 +235,8 @@ def harmonize_many(problems: list[dict], *, max_group: int | None = None,
         active = [i for i, m in enumerate(meta) if not m["done"] and it < m["max_iter"]]
         if not active:
             break
-        groups: dict = {}
-        alone = []
-        for i in active:
-            c = sweep_class(jobs[i])
-            (groups.setdefault(c, []) if c is not None else alone).append(i)
-        for c, members in groups.items():
-            step = max_group or len(members)
-            for g0 in range(0, len(members), step):
-                part = members[g0:g0 + step]
-                if len(part) == 1:
-                    meta[part[0]]["clustering_iters"].append(jobs[part[0]].clustering(it))
-                    continue
-                for i, nc in zip(part, clustering_batched([jobs[i] for i in part], it)):
-                    meta[i]["clustering_iters"].append(nc)
-        for i in alone:
-            meta[i]["clustering_iters"].append(jobs[i].clustering(it))
+        for i, nc in zip(active, clustering_step([jobs[i] for i in active], it, max_group)):
+            meta[i]["clustering_iters"].append(nc)
         with torch.cuda.stream(stream):
             lasts = torch.stack([jobs[i].last_obj.double().reshape(()) for i in active]).cpu()
         for i, v in zip(active, lasts.numpy()):
