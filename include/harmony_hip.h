@@ -228,6 +228,12 @@ typedef struct hz_job {
 size_t hz_batched_workspace_bytes(int n_jobs, int max_iter);
 int hz_clustering_loop_batched(hz_job* jobs, int n_jobs, void* bws, size_t bws_bytes,
                                void* stream);
+/* *eligible = 1 when a job of these dimensions can run in
+ * hz_clustering_loop_batched on the current device (float32, one covariate,
+ * at most 256 clusters and 64 blocks, and a block sweep whose LDS fits), else
+ * 0.  Jobs that are not eligible run through hz_clustering_loop.  No reference
+ * counterpart (the reference integrates one dataset per call). */
+int hz_batched_eligible(const hz_dims* dims, int* eligible);
 
 /* Ridge correction from the ORIGINAL X: lambda_kb (written to lambda_kb_out,
  * n_batches x n_clusters), Z_hat and its L2-normalised rows Z_norm.

@@ -76,6 +76,7 @@ _SIGNATURES = {
          ctypes.POINTER(_i), _vp, _sz, _vp],
     ),
     "hz_batched_workspace_bytes": (_sz, [_i, _i]),
+    "hz_batched_eligible": (_i, [_vp, _vp]),
     "hz_clustering_loop_batched": (_i, [_vp, _i, _vp, _sz, _vp]),
     "hz_correction": (
         _i,

@@ -749,7 +749,8 @@ static SweepArgs make_sweep_args(const Layout& L, const float* Z, float* R, floa
   sw.cnt = L.swcnt;
   sw.trash = L.swtrash;
   // counted block sums: value field below bit 53 holds at most N * 2^s
-  sw.scale_pk = std::ldexp(1.0, std::min(40, 52 - bit_length((unsigned long long)L.n + 1)));
+  // a lane's sum over 4 rows (<= 4) times the scale fits in 32 bits
+  sw.scale_pk = std::ldexp(1.0, std::min(29, 52 - bit_length((unsigned long long)L.n + 1)));
   sw.inv_scale_pk = 1.0 / sw.scale_pk;
   // polls of about a microsecond: a wait that long is a fault
   sw.spin_limit = g_tune[HZ_TUNE_SPIN_LIMIT] > 0 ? (unsigned)g_tune[HZ_TUNE_SPIN_LIMIT] : 1u << 16;
@@ -762,6 +763,15 @@ static void bind_sweep_tab(SweepArgs& sw, const ChunkTab& t) {
   sw.ch_start = t.ch_start;
   sw.ch_len = t.ch_len;
   sw.ch_seg = t.ch_seg;
+}
+
+// Shape rules of the block sweep: 16-row chunks, at most 256 clusters and
+// kSweepMaxBlocks blocks, one covariate, and R small enough for the sweep's
+// store offsets (n K 4 bytes <= kNoRow).
+static bool sweep_rules(const Layout& L) {
+  return L.chunk == 16 && L.K <= 256 && L.pd.nblk <= kSweepMaxBlocks && L.C == 1 &&
+         (unsigned long long)L.n * (unsigned long long)L.K * 4ull <= kNoRow &&
+         g_tune[HZ_TUNE_SWEEP] == 0 && !sweep_broken();
 }
 
 template <typename T>
@@ -888,9 +898,7 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
                          (reinterpret_cast<uintptr_t>(R) % 16 == 0);
     // one 16-row MFMA tile per chunk
     // the sweep keeps one covariate's O/E/penalty per workgroup in LDS
-    use_sw = use_rm && aligned && L.chunk == 16 && L.K <= 256 && L.pd.nblk <= kSweepMaxBlocks &&
-             L.C == 1 && g_tune[HZ_TUNE_SWEEP] == 0 &&
-             !sweep_broken();
+    use_sw = use_rm && aligned && sweep_rules(L);
     sw = make_sweep_args(L, Z, R, O, E, Pr, theta, sigma, tol, stabilized, last_obj);
     cs.Z = Z;
     cs.R = R;
@@ -1100,6 +1108,20 @@ struct BatchJob {
   int G = 1;
 };
 
+// Whether a job of layout L can run in the multi-job block sweep on the
+// current device: float32 state, one covariate, the MFMA row path with
+// 16-row chunks, at most 256 clusters and kSweepMaxBlocks blocks, and a
+// sweep whose LDS fits (at least one co-resident workgroup per CU).
+static bool batch_rules(const Layout& L) {
+  return L.esz == 4 && L.mfma && sweep_rules(L);
+}
+static bool batch_eligible(const Layout& L) {
+  int cap = 0;
+  return batch_rules(L) &&
+         sweep_multi(L, sweep_lds_of(L), nullptr, nullptr, 1, 0, &cap, nullptr) == HZ_OK &&
+         cap > 0;
+}
+
 static int batched_impl(hz_job* jobs, int n, void* bws, size_t bws_bytes, hipStream_t s) {
   HZ_RET_IF(n < 1 || !jobs, HZ_ERR_ARG, "no jobs");
   const int max_iter = jobs[0].max_iter;
@@ -1119,9 +1141,8 @@ static int batched_impl(hz_job* jobs, int n, void* bws, size_t bws_bytes, hipStr
     HZ_TRY(make_layout(&h.dims, h.ws, L));
     const bool aligned = (reinterpret_cast<uintptr_t>(h.Z_norm) % 16 == 0) &&
                          (reinterpret_cast<uintptr_t>(h.R) % 16 == 0);
-    HZ_RET_IF(L.esz != 4 || !L.mfma || L.chunk != 16 || L.K > 256 || L.C != 1 || !aligned ||
-                  L.pd.nblk > kSweepMaxBlocks || g_tune[HZ_TUNE_SWEEP] != 0 || sweep_broken(),
-              HZ_ERR_UNSUPPORTED, "job %d cannot run in the batched block sweep", j);
+    HZ_RET_IF(!aligned || !batch_rules(L), HZ_ERR_UNSUPPORTED,
+              "job %d cannot run in the batched block sweep", j);
     const Layout& L0 = J[0].L;
     HZ_RET_IF(L.prec != L0.prec || L.NT != L0.NT || row_sm(L) != row_sm(L0), HZ_ERR_UNSUPPORTED,
               "job %d has another kernel shape than job 0 (precision, cluster or PC class)", j);
@@ -1879,6 +1900,15 @@ int hz_clustering_loop(const hz_dims* dims, const void* Z_norm, void* R, void* E
                                    sigma, tol, max_iter, seed, stabilized, (double*)last_obj,
                                    n_iter, ws, ws_bytes, s);
   HZ_RET_IF(true, HZ_ERR_ARG, "dtype must be 4 or 8");
+}
+
+int hz_batched_eligible(const hz_dims* dims, int* eligible) {
+  HZ_RET_IF(!dims || !eligible, HZ_ERR_ARG, "null argument");
+  *eligible = 0;
+  Layout L;
+  HZ_TRY(make_layout(dims, nullptr, L));
+  *eligible = batch_eligible(L) ? 1 : 0;
+  return HZ_OK;
 }
 
 size_t hz_batched_workspace_bytes(int n_jobs, int max_iter) {

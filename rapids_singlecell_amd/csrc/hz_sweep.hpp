@@ -188,7 +188,10 @@ constexpr long long kValueMask = (1LL << kCountShift) - 1;
 // Per-workgroup table of each wave's first chunk of every block: segment,
 // length, start and the 64 cell ids (built once at launch, so that the rows of
 // the next block can be fetched without a chain of dependent loads).
-constexpr int kTabStride = 68;  // ints per (block, wave) entry: seg, len, p0, pad, 64 cells
+constexpr int kChunkRows = 16;  // rows per chunk (one MFMA tile)
+constexpr int kTabStride = 4 + kChunkRows;  // ints per (block, wave) entry: seg, len, p0, pad, cells
+// largest R (n x K floats) in bytes: store offsets past it mark dropped stores
+constexpr unsigned kNoRow = 0x7FFFF000u;
 constexpr int kSweepMaxBlocks = 64;
 // rounds (of NW chunks per workgroup) of one block whose chunks are staged in
 // the round table one block ahead; later rounds fetch their chunk inline
@@ -197,23 +200,10 @@ constexpr int kRoundTab = 8;
 template <int PREC, int SM>
 __host__ __device__ inline size_t sweep_lds(int Kp, int B, int K, int nblk) {
   const size_t y = align_up(row_mfma_lds<PREC, SM>(Kp), 16);
-  return y + (size_t)3 * B * K * 4 + (size_t)2 * B * K * 8 + (size_t)2 * K * 8 +
+  return y + (size_t)2 * B * K * 4 + (size_t)B * Kp * 4 + (size_t)2 * B * K * 8 + (size_t)2 * K * 8 +
          (size_t)nblk * (kBlock / kWave) * kTabStride * 4 + (size_t)nblk * 4 * 4 +
          (size_t)nblk * B * kSweepRep * 4 + (size_t)kRoundTab * (kBlock / kWave) * kTabStride * 4 +
          (size_t)2 * B * 4 + 8 + (size_t)kBlock * 8 + 64;
-}
-
-// e^x as 2^(x log2 e) on the hardware exp2, with the rounding error of the
-// product (and of log2 e in FP32) carried into a first-order correction:
-// within a few ulp of expf for the arguments a row sees (|x| < 120).
-__device__ __forceinline__ float exp_rows(float x) {
-  constexpr float kLog2e = 1.44269502162933349609375f;   // log2(e) rounded to FP32
-  constexpr float kLog2eLo = 1.925963033500011e-08f;     // log2(e) - kLog2e
-  constexpr float kLn2 = 0.693147180559945309f;
-  const float t = x * kLog2e;
-  const float e = __builtin_fmaf(x, kLog2eLo, __builtin_fmaf(x, kLog2e, -t));
-  const float y = __builtin_amdgcn_exp2f(t);
-  return __builtin_fmaf(y, e * kLn2, y);
 }
 
 // The sweep of one job run by `G` workgroups, this one being number `wg`.
@@ -224,8 +214,6 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& a, const int wg, con
   constexpr int NA = (PREC == kPrecF32) ? SM : 8 * SM;
   __shared__ int s_last;
   __shared__ double red[NW];
-  __shared__ float s_part[NW][16 * NT];
-  __shared__ int s_seg[NW];
   __shared__ long long s_objq[NW][2];
   Ctrl* ctrl = a.ctrl;
   // written only between launches, so the same in every workgroup; an earlier
@@ -240,8 +228,8 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& a, const int wg, con
   const size_t yb = align_up(row_mfma_lds<PREC, SM>(Kp), 16);
   float* s_O = reinterpret_cast<float*>(smem + yb);
   float* s_E = s_O + BK;
-  float* s_lpen = s_E + BK;  // log of the penalty
-  double* s_do = reinterpret_cast<double*>(s_lpen + BK);  // new-row deltas (update)
+  float* s_lpen = s_E + BK;  // log2 of the penalty, B x Kp (padded columns: -1000)
+  double* s_do = reinterpret_cast<double*>(s_lpen + B * Kp);  // new-row deltas (update)
   // per-block sums of the new rows (fixed point, scale_pk), aliased
   long long* s_dO = reinterpret_cast<long long*>(s_do);
   double* s_di = s_do + BK;                              // old-row deltas of the next block
@@ -287,6 +275,8 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& a, const int wg, con
     s_pr[b] = a.Pr[b];
     s_th[b] = a.theta[b];
   }
+  // every entry finite from the start (rounds without a chunk read batch 0)
+  for (int q = tid; q < B * Kp; q += kBlock) s_lpen[q] = (q % Kp < K) ? 0.f : -1000.f;
   const float* YTs = reinterpret_cast<const float*>(smem);
   const __bf16* YHs = reinterpret_cast<const __bf16*>(smem);
   const __bf16* YLs = YHs + (size_t)Kp * dPS;
@@ -344,7 +334,8 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& a, const int wg, con
     }
 #pragma unroll
     for (int u = 0; u < 8; ++u)
-      if (j0 + u < a.nblk) s_tab[((j0 + u) * NW + wv) * kTabStride + 4 + lane] = cl[u];
+      if (j0 + u < a.nblk && lane < kChunkRows)
+        s_tab[((j0 + u) * NW + wv) * kTabStride + 4 + lane] = cl[u];
   }
   __syncthreads();
 
@@ -413,33 +404,76 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& a, const int wg, con
     }
   };
 
-  // row epilogue of one 16-row tile (kernels_pen.cuh:27-82): lane holds rows 4*lg + r.
-  // exp(term (1 - sim)) * pen is taken as one exponential of its log,
-  // a = term (1 - sim) + log pen; the normalisations use the hardware
-  // reciprocal (1 ulp).  The entropy term x log x of the twice-normalised
-  // value x = e^a / (s s2) uses log x = a - log s - log s2: two logarithms
-  // per row instead of one per value (x log(x + 1e-12) and x log x differ by
-  // less than 1e-12 per value).
-  auto finish_tile = [&](int b, int my_cell, int len, int r0, const floatx4 (&acc)[NT],
-                         float (&dO)[NT]) __attribute__((always_inline)) {
-    float kmf = 0.f, entf = 0.f;
+  // R through a buffer resource: a store whose offset is past the end of R
+  // is dropped by the hardware, so rows past a chunk and padded columns need
+  // no address selection and every round issues the same number of stores.
+  // A round's values are stored at the start of the next round: the wait for
+  // a round's Z rows (which also waits for every store issued before it)
+  // then comes a whole epilogue after the stores.
+  // Row offsets are below kNoRow (the host checks n K 4 <= kNoRow); a dropped
+  // row gets kNoRow, a padded column 2^31, and every sum of the two stays
+  // below 2^32 and at or past the end of R.
+  const __amdgpu_buffer_rsrc_t r_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+      static_cast<void*>(a.R), (short)0, (int)((unsigned)a.n * (unsigned)K * 4u), 0x00020000);
+  unsigned coff[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) coff[t] = (16 * t + lr < K) ? 4u * (unsigned)(16 * t + lr) : 0x80000000u;
+  const unsigned row_bytes = 4u * (unsigned)K;
+  constexpr float kLog2eF = 1.44269502162933349609375f;
+  constexpr float kLn2F = 0.693147180559945309f;
+  const float term2 = a.term * kLog2eF;
+  const float scale_pk_f = (float)a.scale_pk;
+
+  // the chunk of one round: segment (-1: none), length and this lane's cell
+  // (lane l < kChunkRows holds row l)
+  struct Chunk {
+    int seg, len, cell;
+  };
+  // per-lane fixed-point column sums of the wave's current segment
+  unsigned long long qacc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) qacc[t] = 0ull;
+
+  // row epilogue of one 16-row tile (kernels_pen.cuh:27-82): lane holds rows
+  // 4*lg + r at columns 16 t + lr.
+  // exp(term (1 - sim)) * pen is one power of two, 2^a with
+  // a = term log2(e) (1 - sim) + log2 pen; padded columns carry
+  // log2 pen = -1000, so their values are 0.  The normalisations use the
+  // hardware reciprocal (1 ulp).  The entropy term x log x of the
+  // twice-normalised value x = 2^a / (s s2) uses log2 x = a - log2 s - log2 s2:
+  // two logarithms per row instead of one per value (x log(x + 1e-12) and
+  // x log x differ by less than 1e-12 per value).
+  // Column sums: each lane's sum over its 4 rows is rounded to fixed point on
+  // its own (scale_pk <= 2^29: the 4 rows fit in 32 bits) and added to the
+  // lane's 64-bit accumulators: integer sums, independent of which workgroup
+  // and wave process the chunk.
+  auto finish_tile = [&](const Chunk& c, int sb, const floatx4 (&acc)[NT], float (&V)[NT][4],
+                         unsigned (&roff)[4]) __attribute__((always_inline)) {
+    const bool mine = c.seg >= 0;
+    const int bb = mine ? c.seg - sb : 0;
     float lp[NT];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      const int k = 16 * t + lr;
-      lp[t] = s_lpen[b * K + min(k, K - 1)];
-    }
+    for (int t = 0; t < NT; ++t) lp[t] = s_lpen[bb * Kp + 16 * t + lr];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int rr = r0 + 4 * lg + r;
-      const int rcell = __shfl(my_cell, rr & 63, kWave);
+      const int rr = 4 * lg + r;
+      const int rc = __shfl(c.cell, rr, kWave);
+      roff[r] = (rr < c.len) ? (unsigned)rc * row_bytes : kNoRow;  // len = 0 without a chunk
+    }
+    float kmf = 0.f, entf = 0.f;
+    float dO[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) dO[t] = 0.f;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const float m = (roff[r] != kNoRow) ? 1.f : 0.f;
       float v[NT], arg[NT], om[NT];
       float s = 0.f;
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
         om[t] = 1.f - acc[t][r];
-        arg[t] = __builtin_fmaf(a.term, om[t], lp[t]);
-        v[t] = (16 * t + lr < K) ? exp_rows(arg[t]) : 0.f;
+        arg[t] = __builtin_fmaf(term2, om[t], lp[t]);
+        v[t] = __builtin_amdgcn_exp2f(arg[t]);
         s += v[t];
       }
       s = sum16(s);
@@ -456,21 +490,14 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& a, const int wg, con
       const float s2_lane = s2;
       s2 = sum16(s2);
       const float inv2 = __builtin_amdgcn_rcpf(s2);
-      const float lrow = -(__logf(s) + __logf(s2));
-      // every lane stores every value: rows past the chunk and padded
-      // columns go to a scratch row, so the store count is fixed and later
-      // waits on older loads need not wait for these stores
-      const bool row_ok = rr < len;
-      const float m = row_ok ? 1.f : 0.f;
-      float* dst = row_ok ? a.R + (unsigned)rcell * (unsigned)K : a.trash;
+      const float lrow = -(__builtin_amdgcn_logf(s) + __builtin_amdgcn_logf(s2));
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
-        const int k = 16 * t + lr;
-        (k < K ? dst : a.trash)[k] = v[t];
+        V[t][r] = v[t];
         dO[t] = __builtin_fmaf(m, v[t], dO[t]);
       }
-      // k-means term sum v 2 (1 - sim); entropy sum x log x with
-      // x = v / s2 and log x = arg - log s - log s2
+      // k-means term sum v 2 (1 - sim); entropy sum x log2 x with
+      // x = v / s2 and log2 x = a - log2 s - log2 s2 (times ln 2 below)
       kmf = __builtin_fmaf(2.f * m, kv, kmf);
       entf = __builtin_fmaf(m * inv2, __builtin_fmaf(lrow, s2_lane, va), entf);
     }
@@ -479,26 +506,47 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& a, const int wg, con
     // the tiles are spread over workgroups and waves
     constexpr float kObjQ = 16777216.f;
     kmq += (long long)(int)__builtin_rintf(kmf * kObjQ);
-    entq += (long long)(int)__builtin_rintf(entf * kObjQ);
+    entq += (long long)(int)__builtin_rintf(entf * kLn2F * kObjQ);
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+      qacc[t] += (unsigned long long)(unsigned)__builtin_rintf(dO[t] * scale_pk_f);
+  };
+  auto store_rows = [&](const float (&V)[NT][4], const unsigned (&roff)[4]) __attribute__((always_inline)) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, V[t][r]), r_rsrc,
+                                              (int)(roff[r] + coff[t]), 0, 0);
+    }
+  };
+  // adds of the wave's accumulators to segment seg's sums (LDS atomics; the
+  // four lane groups add their own row partials)
+  auto flush_q = [&](int seg, int sb) __attribute__((always_inline)) {
+    unsigned long long* dst = reinterpret_cast<unsigned long long*>(s_dO) + (seg - sb) * K;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const int k = 16 * t + lr;
+      if (k < K) atomicAdd(dst + k, qacc[t]);
+      qacc[t] = 0ull;
+    }
   };
 
-  // wave-carried prefetch of the first chunk of the next block: its table
-  // entry, its Z rows (loads issued one block ahead, in zpre) and its
-  // similarity tile (n_sim)
-  int n_seg = -1, n_len = 0, n_cell = -1;
-  floatx4 n_sim[NT];
-  float zpre[NA];
+  // prefetch of the next block's first round: its chunk and similarity tile
+  // (Z rows loaded one block ahead, in zpre)
+  Chunk n_c0{-1, 0, -1};
+  floatx4 tA[NT];
+  float zpre[NA], zA[NA];
   auto issue_z = [&](int jn) __attribute__((always_inline)) {
     const int* e = s_tab + (jn * NW + wv) * kTabStride;
-    load_z(e[4 + lane], e[1], 0, zpre);
+    load_z(e[4 + (lane & (kChunkRows - 1))], e[1], 0, zpre);
   };
   auto take_prefetch = [&](int jn) __attribute__((always_inline)) {
     const int* e = s_tab + (jn * NW + wv) * kTabStride;
-    n_seg = e[0];
-    n_len = e[1];
-    n_cell = e[4 + lane];
-    sim_mfma(zpre, n_sim);
+    n_c0 = Chunk{e[0], e[1], e[4 + (lane & (kChunkRows - 1))]};
+    sim_mfma(zpre, tA);
   };
+
   // Per-(batch, cluster) reads use a column-major thread mapping: thread tid
   // handles cluster ck = tid % K and batches cg, cg + GB, ... (cg = tid / K),
   // so that it can also sum its column without another pass over LDS.
@@ -520,127 +568,95 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& a, const int wg, con
   };
 
   // chunks of rounds 1 .. kRoundTab-1 of block jn (round 0 is in s_tab):
-  // entry (r, wave) holds segment, length, start and the 64 cell ids of the
+  // entry (r, wave) holds segment, length, start and the cell ids of the
   // wave's chunk lo + r*NW + wave.  Two rounds of independent loads, issued
   // while the other workgroups' sums of the current block arrive.
   auto stage_rounds = [&](int jn) __attribute__((always_inline)) {
     int lo, hi;
     wg_range(jn, lo, hi);
-    if (lo + NW >= hi) return;  // a single round
-    if (lane < kRoundTab && lane >= 1) {
-      const int c = lo + lane * NW + wv;
-      int* e = s_rtab + (lane * NW + wv) * kTabStride;
-      if (c < hi) {
-        e[0] = a.ch_seg[c];
-        e[1] = a.ch_len[c];
-        e[2] = a.ch_start[c];
-      } else {
-        e[0] = -1;
-        e[1] = 0;
-        e[2] = 0;
+    if (lo + NW < hi) {  // more than one round
+      if (lane < kRoundTab && lane >= 1) {
+        const int c = lo + lane * NW + wv;
+        int* e = s_rtab + (lane * NW + wv) * kTabStride;
+        if (c < hi) {
+          e[0] = a.ch_seg[c];
+          e[1] = a.ch_len[c];
+          e[2] = a.ch_start[c];
+        } else {
+          e[0] = -1;
+          e[1] = 0;
+          e[2] = 0;
+        }
       }
-    }
-    __syncthreads();
-    int cl[kRoundTab];
+      __syncthreads();
+      int cl[kRoundTab];
 #pragma unroll
-    for (int r = 1; r < kRoundTab; ++r) {
-      const int* e = s_rtab + (r * NW + wv) * kTabStride;
-      const int len = e[1];
-      const int x = a.order[(lane < len) ? e[2] + lane : 0];  // order[0] is always valid
-      cl[r] = (lane < len) ? x : -1;
-    }
+      for (int r = 1; r < kRoundTab; ++r) {
+        const int* e = s_rtab + (r * NW + wv) * kTabStride;
+        const int len = e[1];
+        const int x = a.order[(lane < len) ? e[2] + lane : 0];  // order[0] is always valid
+        cl[r] = (lane < len) ? x : -1;
+      }
+      if (lane < kChunkRows) {
 #pragma unroll
-    for (int r = 1; r < kRoundTab; ++r) s_rtab[(r * NW + wv) * kTabStride + 4 + lane] = cl[r];
-    __syncthreads();
+        for (int r = 1; r < kRoundTab; ++r) s_rtab[(r * NW + wv) * kTabStride + 4 + lane] = cl[r];
+      }
+      __syncthreads();
+    }
   };
 
-  // rows of block j for this workgroup (new R; per-block sums into s_dO,
-  // each chunk's sum rounded to fixed point on its own, so the workgroup's
-  // totals do not depend on which chunks it was given).
-  // Round 0 uses the prefetched tile; the Z rows of round r+1 are loaded
-  // while round r is normalised.
+  // chunk of round r of this workgroup's range [lo, hi) for this wave
+  auto round_chunk = [&](int lo, int hi, int r) __attribute__((always_inline)) {
+    Chunk c;
+    if (r < kRoundTab) {
+      const int* e = s_rtab + (r * NW + wv) * kTabStride;
+      c.seg = e[0];
+      c.len = e[1];
+      c.cell = e[4 + (lane & (kChunkRows - 1))];
+    } else {
+      const int q = lo + r * NW + wv;
+      c.seg = (q < hi) ? a.ch_seg[q] : -1;
+      c.len = (q < hi) ? a.ch_len[q] : 0;
+      const int p0 = (q < hi) ? a.ch_start[q] : 0;
+      const int x = a.order[(lane < c.len) ? p0 + lane : 0];
+      c.cell = (lane < c.len) ? x : -1;
+    }
+    return c;
+  };
+
+  // rows of block j for this workgroup (new R; per-block sums into s_dO).
+  // Round r: the stores of round r-1's values, the loads of round r+1's Z
+  // rows, the epilogue of round r's tile, then (Z rows long arrived) the
+  // MFMAs of round r+1's tile.
   auto rows_block = [&](int j) __attribute__((always_inline)) {
     int lo, hi;
     wg_range(j, lo, hi);
     const int sb = j * B;
     for (int q = tid; q < BK; q += kBlock) s_dO[q] = 0;
-    float zc[NA];
-    int cseg = -1, clen = 0, ccell = -1;
-    auto round_chunk = [&](int r, int& seg, int& len, int& cell) __attribute__((always_inline)) {
-      const int c = lo + r * NW + wv;
-      if (r < kRoundTab) {
-        const int* e = s_rtab + (r * NW + wv) * kTabStride;
-        seg = e[0];
-        len = e[1];
-        cell = e[4 + lane];
-      } else {
-        seg = (c < hi) ? a.ch_seg[c] : -1;
-        len = (c < hi) ? a.ch_len[c] : 0;
-        const int p0 = (c < hi) ? a.ch_start[c] : 0;
-        const int x = a.order[(lane < len) ? p0 + lane : 0];
-        cell = (lane < len) ? x : -1;
+    __syncthreads();
+    Chunk c0 = n_c0;
+    int qseg = -1;
+    float V[NT][4];
+    unsigned roff[4] = {kNoRow, kNoRow, kNoRow, kNoRow};  // nothing pending
+    for (int cb = lo, r = 0; cb < hi; cb += NW, ++r) {
+      store_rows(V, roff);
+      const bool more = cb + NW < hi;  // workgroup-uniform
+      Chunk c1{-1, 0, -1};
+      if (more) {
+        c1 = round_chunk(lo, hi, r + 1);
+        load_z(c1.cell, c1.len, 0, zA);
       }
-    };
-    int r = 0;
-    for (int cb = lo; cb < hi; cb += NW, ++r) {
-      const bool mine = cb + wv < hi;  // wave-uniform
-      // this round's chunk: round 0 from the block prefetch (tile in n_sim),
-      // later rounds from the previous round (Z rows in zc)
-      int seg, len, cell;
-      floatx4 acc[NT];
-      if (r == 0) {
-        seg = n_seg;
-        len = n_len;
-        cell = n_cell;
-      } else {
-        seg = cseg;
-        len = clen;
-        cell = ccell;
-        if (mine) sim_mfma(zc, acc);
+      if (c0.seg >= 0 && c0.seg != qseg) {  // wave-uniform
+        if (qseg >= 0) flush_q(qseg, sb);
+        qseg = c0.seg;
       }
-      // next round's chunk; its Z rows load into zc (whose MFMAs are issued)
-      // while this round is normalised.  The loads are unconditional: past
-      // the last round they read row 0.
-      if (cb + NW < hi) {
-        round_chunk(r + 1, cseg, clen, ccell);
-      } else {
-        cseg = -1;
-        clen = 0;
-        ccell = -1;
-      }
-      load_z(ccell, clen, 0, zc);
-      if (mine) {
-        float dO[NT];
-#pragma unroll
-        for (int t = 0; t < NT; ++t) dO[t] = 0.f;
-        const int b = seg - sb;
-        if (r == 0)
-          finish_tile(b, cell, len, 0, n_sim, dO);
-        else
-          finish_tile(b, cell, len, 0, acc, dO);
-#pragma unroll
-        for (int t = 0; t < NT; ++t) {
-          float x = dO[t];
-          x += __shfl_xor(x, 16, kWave);
-          x += __shfl_xor(x, 32, kWave);
-          const int k = 16 * t + lr;
-          if (lg == 0 && k < K) s_part[wv][k] = x;
-        }
-      } else {
-        seg = -1;
-      }
-      if (lane == 0) s_seg[wv] = seg;
-      __syncthreads();
-      // fold this round's chunk sums in fixed wave order (deterministic)
-      for (int k = tid; k < K; k += kBlock) {
-#pragma unroll
-        for (int w = 0; w < NW; ++w) {
-          const int sg = s_seg[w];
-          if (sg >= 0) s_dO[(sg - sb) * K + k] += llrint((double)s_part[w][k] * a.scale_pk);
-        }
-      }
-      __syncthreads();
+      finish_tile(c0, sb, tA, V, roff);
+      if (more) sim_mfma(zA, tA);
+      c0 = c1;
     }
+    store_rows(V, roff);
+    if (qseg >= 0) flush_q(qseg, sb);
+    __syncthreads();
   };
 
   // fixed-point adds of this workgroup's batch x cluster sums of block j
@@ -778,7 +794,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& a, const int wg, con
       // the penalty is needed only for the batches this workgroup processes next
       if (next && b >= pbl && b <= pbh) {
         const float denom = a.stabilized ? (o + e + 1.f) : (o + 1.f);
-        s_lpen[q] = s_th[b] * __builtin_logf((e + 1.f) / denom);
+        s_lpen[b * Kp + k] = s_th[b] * (__builtin_logf((e + 1.f) / denom) * kLog2eF);
       }
     }
     __syncthreads();

@@ -71,16 +71,21 @@ def _sm_class(d: int, precision: int) -> int:
 
 def sweep_class(job: HarmonyJob):
     """Kernel shape of a job's block sweep, or None when the job cannot run
-    in the batched sweep (the library checks the same rules)."""
-    if job.dtype != torch.float32 or job.n_covariates != 1 or job.n_clusters > 256:
-        return None
-    nblk = -(-job.n_cells // job.block_size)
-    if nblk > 64:
-        return None
-    if job.X.shape[0] * max(job.n_clusters, job.n_pcs + 1) >= 2 ** 31:
-        return None
-    return (job.precision, _nt_class(job.n_clusters), _sm_class(job.n_pcs, job.precision),
-            job.max_iter_clustering)
+    in the batched sweep (``hz_batched_eligible``: float32, one covariate,
+    at most 256 clusters and 64 blocks, and a sweep whose LDS fits)."""
+    cached = getattr(job, "_sweep_class", False)
+    if cached is not False:
+        return cached
+    c = None
+    if job.dtype == torch.float32 and job.n_covariates == 1:
+        ok = ctypes.c_int(0)
+        _native.check(_native.load().hz_batched_eligible(ctypes.byref(job.dims), ctypes.byref(ok)),
+                      "hz_batched_eligible")
+        if ok.value:
+            c = (job.precision, _nt_class(job.n_clusters), _sm_class(job.n_pcs, job.precision),
+                 job.max_iter_clustering)
+    job._sweep_class = c
+    return c
 
 
 _bws_cache: dict = {}

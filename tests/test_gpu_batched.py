@@ -64,7 +64,9 @@ def test_batched_equals_alone(device, precision):
 def test_batched_mixed_classes_and_fallback(device):
     probs = [_problem(20_000, 30, 4, 60, 11), _problem(30_000, 50, 5, 100, 12),
              _problem(15_000, 30, 3, 50, 13), _problem(8_000, 20, 3, 40, 14, dtype=np.float64),
-             _problem(12_000, 50, 4, 100, 15, precision="split_bf16")]
+             _problem(12_000, 50, 4, 100, 15, precision="split_bf16"),
+             # 32 batches x 100 clusters at 100 PCs: the sweep's LDS does not fit
+             _problem(20_000, 100, 32, 100, 16)]
     for p in probs:  # fixed iteration counts
         p.update(tol_harmony=float("-inf"), tol_clustering=float("-inf"))
     got = harmonize_many(probs)
@@ -120,3 +122,23 @@ def test_batched_rejects_mixed_shapes(device):
                                         ctypes.c_void_p(0))
     assert rc == -4
     assert b"kernel shape" in lib.hz_last_error()
+
+
+def test_batched_eligibility(device):
+    """``hz_batched_eligible``: a job whose block sweep needs more LDS than a
+    CU has (32 batches x 100 clusters at 100 PCs) runs alone; float32 jobs of
+    smaller shapes are eligible."""
+    from rapids_singlecell_amd.preprocessing._harmony._batch import sweep_class
+    from rapids_singlecell_amd.preprocessing._harmony._job import HarmonyJob
+
+    def job(d, B, K):
+        X, obs = synthetic_pcs(20_000, d, 5, B, seed=d + B)
+        codes = np.asarray(obs["batch"].cat.codes.values, dtype=np.int32)
+        return HarmonyJob(torch.as_tensor(X, device="cuda"), codes, B, n_clusters=K,
+                          theta_np=np.full(B, 2.0, np.float32),
+                          N_b_np=np.bincount(codes, minlength=B).astype(np.float32),
+                          max_iter_clustering=4)
+
+    assert sweep_class(job(100, 32, 100)) is None
+    assert sweep_class(job(50, 5, 100)) is not None
+    assert sweep_class(job(30, 32, 40)) is not None
