@@ -189,6 +189,7 @@ constexpr long long kValueMask = (1LL << kCountShift) - 1;
 // length, start and the 64 cell ids (built once at launch, so that the rows of
 // the next block can be fetched without a chain of dependent loads).
 constexpr int kChunkRows = 16;  // rows per chunk (one MFMA tile)
+constexpr int kTrStride = 20;   // floats per row of a wave's 16 x 16 transpose tile
 constexpr int kTabStride = 4 + kChunkRows;  // ints per (block, wave) entry: seg, len, p0, pad, cells
 // largest R (n x K floats) in bytes: store offsets past it mark dropped stores
 constexpr unsigned kNoRow = 0x7FFFF000u;
@@ -203,7 +204,8 @@ __host__ __device__ inline size_t sweep_lds(int Kp, int B, int K, int nblk) {
   return y + (size_t)2 * B * K * 4 + (size_t)B * Kp * 4 + (size_t)2 * B * K * 8 + (size_t)2 * K * 8 +
          (size_t)nblk * (kBlock / kWave) * kTabStride * 4 + (size_t)nblk * 4 * 4 +
          (size_t)nblk * B * kSweepRep * 4 + (size_t)kRoundTab * (kBlock / kWave) * kTabStride * 4 +
-         (size_t)2 * B * 4 + 8 + (size_t)kBlock * 8 + 64;
+         (size_t)2 * B * 4 + 8 + (size_t)kBlock * 8 + 64 +
+         (size_t)(kBlock / kWave) * kChunkRows * kTrStride * 4;
 }
 
 // The sweep of one job run by `G` workgroups, this one being number `wg`.
@@ -211,7 +213,8 @@ template <int PREC, int NT, int SM>
 __device__ __forceinline__ void sweep_body(const SweepArgs& a, const int wg, const int G) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int NW = kBlock / kWave;
-  constexpr int NA = (PREC == kPrecF32) ? SM : 8 * SM;
+  // floats of Z per lane and chunk
+  constexpr int NZ = (PREC == kPrecF32) ? 4 * ((SM + 3) / 4) : 8 * SM;
   __shared__ int s_last;
   __shared__ double red[NW];
   __shared__ long long s_objq[NW][2];
@@ -243,6 +246,9 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& a, const int wg, con
   float* s_th = s_pr + B;                                                        // theta_b
   double* s_colp = reinterpret_cast<double*>(  // kBlock column partials
       smem + align_up((size_t)(reinterpret_cast<unsigned char*>(s_th + B) - smem), 8));
+  // this wave's 16 x 16 transpose tile (rows of kTrStride floats, 16-byte aligned)
+  float* s_tr = reinterpret_cast<float*>(smem + align_up((size_t)(
+      reinterpret_cast<unsigned char*>(s_colp + kBlock) - smem), 16)) + wv * kChunkRows * kTrStride;
 
   // stage the normalised centroids (same padded layouts as the row kernel)
   const int KpS = Kp + 16;
@@ -339,31 +345,40 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& a, const int wg, con
   }
   __syncthreads();
 
-  // similarities of 16 rows (r0 .. r0+15 of a chunk) on matrix cores
-  // A operand of one 16-row tile: row r0 + lr of the chunk.  Nothing is
-  // masked after the load, so the loads stay in flight until the MFMAs use
-  // them: a row past len reads row 0 (its results are never stored) and a
-  // column c >= d reads column d-1, which meets the zero rows c >= d of the
-  // staged centroids.
-  auto load_z = [&](int my_cell, int len, int r0, float (&za)[NA]) __attribute__((always_inline)) {
-    const int zc = __shfl(my_cell, (r0 + lr) & 63, kWave);
-    const bool zvalid = r0 + lr < len;
-    const unsigned zbase = (unsigned)(zvalid ? zc : 0) * (unsigned)d;
+  // similarities of the 16 rows of a chunk on matrix cores.
+  // A operand: row lr of the chunk, 16 bytes per load through a buffer
+  // resource (loads past the end of Z return 0).  Each lane takes contiguous
+  // columns: in split-BF16 mode columns 32 s + 8 lg .. +7 of k-step s, in FP32
+  // mode columns lg SM .. lg SM + SM - 1 (k-step s uses column lg SM + s).
+  // Nothing is masked after the load, so the loads stay in flight until the
+  // MFMAs use them: a row past len reads row 0 (its results are never
+  // stored) and a column c >= d reads the next row or zeros, which meet the
+  // zero rows c >= d of the staged centroids.
+  const __amdgpu_buffer_rsrc_t z_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<float*>(a.Z), (short)0, (int)((unsigned)a.n * (unsigned)d * 4u), 0x00020000);
+  auto load_z = [&](int my_cell, int len, float (&za)[NZ]) __attribute__((always_inline)) {
+    const int zc = __shfl(my_cell, lr, kWave);
+    const unsigned zoff = (unsigned)(lr < len ? zc : 0) * (unsigned)d * 4u;
 #pragma unroll
-    for (int i = 0; i < NA; ++i) {
-      const int cc = (PREC == kPrecF32) ? 4 * i + lg : 32 * (i / 8) + 8 * lg + (i % 8);
-      za[i] = a.Z[zbase + (unsigned)min(cc, d - 1)];
+    for (int q = 0; q < NZ / 4; ++q) {
+      const int col = (PREC == kPrecF32) ? lg * SM + 4 * q : 32 * (q / 2) + 8 * lg + 4 * (q % 2);
+      // (the whole vector is bit-cast: element-wise casts of the loaded
+      // vector let the optimiser shrink the load to its first dword)
+      const floatx4 v = __builtin_bit_cast(
+          floatx4, __builtin_amdgcn_raw_buffer_load_b128(z_rsrc, (int)(zoff + 4u * (unsigned)col), 0, 0));
+#pragma unroll
+      for (int e = 0; e < 4; ++e) za[4 * q + e] = v[e];
     }
   };
   // MFMA chain of one tile; the B fragments (centroids in LDS) of the next
   // k-step are read before the MFMAs of the current one
-  auto sim_mfma = [&](const float (&za)[NA], floatx4 (&acc)[NT]) __attribute__((always_inline)) {
+  auto sim_mfma = [&](const float (&za)[NZ], floatx4 (&acc)[NT]) __attribute__((always_inline)) {
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
     if (PREC == kPrecF32) {
       float bv[2][NT];
       auto rd = [&](int s, int bsel) __attribute__((always_inline)) {
-        const int cc = 4 * s + lg;
+        const int cc = lg * SM + s;
 #pragma unroll
         for (int t = 0; t < NT; ++t) bv[bsel][t] = YTs[cc * KpS + 16 * t + lr];
       };
@@ -407,18 +422,20 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& a, const int wg, con
   // R through a buffer resource: a store whose offset is past the end of R
   // is dropped by the hardware, so rows past a chunk and padded columns need
   // no address selection and every round issues the same number of stores.
-  // A round's values are stored at the start of the next round: the wait for
-  // a round's Z rows (which also waits for every store issued before it)
-  // then comes a whole epilogue after the stores.
+  // Each 16 x 16 tile of values is transposed through the wave's LDS tile, so
+  // that lane l holds columns 4 (l % 4) .. +3 of row l / 4 and stores them
+  // with one 16-byte store (four 4-byte stores when K is not a multiple of
+  // 4, which keeps every store inside its row).  A round's values are stored
+  // at the start of the next round, after the loads of the next Z rows: the
+  // wait for those loads then need not wait for these stores.
   // Row offsets are below kNoRow (the host checks n K 4 <= kNoRow); a dropped
   // row gets kNoRow, a padded column 2^31, and every sum of the two stays
   // below 2^32 and at or past the end of R.
   const __amdgpu_buffer_rsrc_t r_rsrc = __builtin_amdgcn_make_buffer_rsrc(
       static_cast<void*>(a.R), (short)0, (int)((unsigned)a.n * (unsigned)K * 4u), 0x00020000);
-  unsigned coff[NT];
-#pragma unroll
-  for (int t = 0; t < NT; ++t) coff[t] = (16 * t + lr < K) ? 4u * (unsigned)(16 * t + lr) : 0x80000000u;
+  const int tq = lane & 3;  // column quad of the transposed layout
   const unsigned row_bytes = 4u * (unsigned)K;
+  const bool vec4 = K % 4 == 0;  // 16-byte stores stay inside a row
   constexpr float kLog2eF = 1.44269502162933349609375f;
   constexpr float kLn2F = 0.693147180559945309f;
   const float term2 = a.term * kLog2eF;
@@ -447,26 +464,28 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& a, const int wg, con
   // its own (scale_pk <= 2^29: the 4 rows fit in 32 bits) and added to the
   // lane's 64-bit accumulators: integer sums, independent of which workgroup
   // and wave process the chunk.
-  auto finish_tile = [&](const Chunk& c, int sb, const floatx4 (&acc)[NT], float (&V)[NT][4],
-                         unsigned (&roff)[4]) __attribute__((always_inline)) {
+  auto finish_tile = [&](const Chunk& c, int sb, const floatx4 (&acc)[NT], floatx4 (&F4)[NT],
+                         unsigned& roffl) __attribute__((always_inline)) {
     const bool mine = c.seg >= 0;
     const int bb = mine ? c.seg - sb : 0;
     float lp[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) lp[t] = s_lpen[bb * Kp + 16 * t + lr];
+    float m4[4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int rr = 4 * lg + r;
-      const int rc = __shfl(c.cell, rr, kWave);
-      roff[r] = (rr < c.len) ? (unsigned)rc * row_bytes : kNoRow;  // len = 0 without a chunk
+    for (int r = 0; r < 4; ++r) m4[r] = (4 * lg + r < c.len) ? 1.f : 0.f;  // len = 0 without a chunk
+    {
+      const int rc = __shfl(c.cell, lane >> 2, kWave);
+      roffl = ((lane >> 2) < c.len) ? (unsigned)rc * row_bytes : kNoRow;
     }
+    float V[NT][4];
     float kmf = 0.f, entf = 0.f;
     float dO[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) dO[t] = 0.f;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const float m = (roff[r] != kNoRow) ? 1.f : 0.f;
+      const float m = m4[r];
       float v[NT], arg[NT], om[NT];
       float s = 0.f;
 #pragma unroll
@@ -510,14 +529,35 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& a, const int wg, con
 #pragma unroll
     for (int t = 0; t < NT; ++t)
       qacc[t] += (unsigned long long)(unsigned)__builtin_rintf(dO[t] * scale_pk_f);
+    // transposes: lane (lr, lg) writes rows 4 lg + r of column lr, lane l
+    // reads row l / 4, columns 4 (l % 4) .. +3 (in-order LDS within the wave)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) s_tr[(4 * lg + r) * kTrStride + lr] = V[t][r];
+      F4[t] = *reinterpret_cast<const floatx4*>(s_tr + (lane >> 2) * kTrStride + 4 * tq);
+    }
   };
-  auto store_rows = [&](const float (&V)[NT][4], const unsigned (&roff)[4]) __attribute__((always_inline)) {
+  auto store_rows = [&](const floatx4 (&F4)[NT], unsigned roffl) __attribute__((always_inline)) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int t = 0; t < NT; ++t) {
+      const int c0 = 16 * t + 4 * tq;
+      const unsigned o = roffl + 4u * (unsigned)c0;
+      // (the vector goes to the buffer builtin whole, by bit cast: element
+      // by element, later elements can be replaced by the first)
+      if (vec4) {
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, F4[t]), r_rsrc,
+                                               (int)(c0 < K ? o : 0x80000000u), 0, 0);
+      } else {
+        // plain stores, one per valid element (the buffer builtins misread
+        // the elements of a vector passed one at a time)
+        if (roffl != kNoRow) {
+          float* dst = a.R + (roffl >> 2) + c0;
 #pragma unroll
-      for (int t = 0; t < NT; ++t)
-        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, V[t][r]), r_rsrc,
-                                              (int)(roff[r] + coff[t]), 0, 0);
+          for (int e = 0; e < 4; ++e)
+            if (c0 + e < K) dst[e] = F4[t][e];
+        }
+      }
     }
   };
   // adds of the wave's accumulators to segment seg's sums (LDS atomics; the
@@ -536,10 +576,10 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& a, const int wg, con
   // (Z rows loaded one block ahead, in zpre)
   Chunk n_c0{-1, 0, -1};
   floatx4 tA[NT];
-  float zpre[NA], zA[NA];
+  float zpre[NZ], zA[NZ];
   auto issue_z = [&](int jn) __attribute__((always_inline)) {
     const int* e = s_tab + (jn * NW + wv) * kTabStride;
-    load_z(e[4 + (lane & (kChunkRows - 1))], e[1], 0, zpre);
+    load_z(e[4 + (lane & (kChunkRows - 1))], e[1], zpre);
   };
   auto take_prefetch = [&](int jn) __attribute__((always_inline)) {
     const int* e = s_tab + (jn * NW + wv) * kTabStride;
@@ -636,25 +676,25 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& a, const int wg, con
     __syncthreads();
     Chunk c0 = n_c0;
     int qseg = -1;
-    float V[NT][4];
-    unsigned roff[4] = {kNoRow, kNoRow, kNoRow, kNoRow};  // nothing pending
+    floatx4 F4[NT];
+    unsigned roffl = kNoRow;  // nothing pending
     for (int cb = lo, r = 0; cb < hi; cb += NW, ++r) {
-      store_rows(V, roff);
       const bool more = cb + NW < hi;  // workgroup-uniform
       Chunk c1{-1, 0, -1};
       if (more) {
         c1 = round_chunk(lo, hi, r + 1);
-        load_z(c1.cell, c1.len, 0, zA);
+        load_z(c1.cell, c1.len, zA);
       }
+      store_rows(F4, roffl);
       if (c0.seg >= 0 && c0.seg != qseg) {  // wave-uniform
         if (qseg >= 0) flush_q(qseg, sb);
         qseg = c0.seg;
       }
-      finish_tile(c0, sb, tA, V, roff);
+      finish_tile(c0, sb, tA, F4, roffl);
       if (more) sim_mfma(zA, tA);
       c0 = c1;
     }
-    store_rows(V, roff);
+    store_rows(F4, roffl);
     if (qseg >= 0) flush_q(qseg, sb);
     __syncthreads();
   };

@@ -403,6 +403,30 @@ def test_clustering_sweep_vs_per_block(device, tuning, precision, stabilized):
 
 
 @pytest.mark.parametrize("precision", ["native", "split_bf16"])
+@pytest.mark.parametrize("K,d", [(50, 50), (75, 30), (64, 17), (33, 64), (130, 50)])
+def test_harmonize_cluster_counts(device, precision, K, d):
+    """Cluster counts that are not multiples of 4 (the block sweep stores R
+    rows with 4-byte stores there, 16-byte stores otherwise), an odd PC count,
+    and K > 128 (16 column tiles): agreement with the FP64 oracle at fixed
+    iteration counts."""
+    n, B = 12000, 5
+    X, obs = synthetic_pcs(n, d, 12, B, seed=K + d)
+    X = X.astype(np.float32)
+    codes, cats, _ = ho.batch_codes(obs, "batch")
+    Z64 = ho.l2_normalize(X.astype(np.float64))
+    Y0 = Z64[np.random.default_rng(K).choice(n, K, replace=False)]
+    kw = dict(n_clusters=K, max_iter_harmony=2, max_iter_clustering=3, random_state=3,
+              tol_harmony=float("-inf"), tol_clustering=float("-inf"))
+    Zg = harmonize(_t(X, device), obs, "batch", init_centroids=Y0.astype(np.float32),
+                   precision=precision, **kw).cpu().numpy()
+    Zr = ho.harmonize(X.astype(np.float64), codes, len(cats), Y0=Y0, **kw)
+    assert np.all(np.isfinite(Zg))
+    e = rel(Zg, Zr)
+    print(f"\nK={K} d={d} {precision}: rel err vs FP64 oracle = {e:.3e}", flush=True)
+    assert e < (1e-4 if precision == "native" else 1e-3), e
+
+
+@pytest.mark.parametrize("precision", ["native", "split_bf16"])
 @pytest.mark.parametrize("n,d,B", [(5395, 30, 7), (5836, 100, 25), (11227, 30, 28), (29263, 50, 32)])
 def test_harmonize_shapes(device, precision, n, d, B):
     """Default-K jobs of the served-form distribution (LDS sizes around the
