@@ -36,8 +36,12 @@
 
 namespace hz {
 
-// replicas of the block accumulators written by the sweep (<= kRep)
-constexpr int kSweepRep = 4;
+// replicas of the block accumulators written by the sweep (<= kRep).  One:
+// every workgroup reads every entry of each block's sums, so replicas
+// multiply the bytes all workgroups pull through the fabric per block (the
+// adds themselves are fire-and-forget).  The two objective accumulators, to
+// which every workgroup adds once at the end, keep kRep replicas.
+constexpr int kSweepRep = 1;
 
 // ---------------------------------------------------------------- old-row sums
 
@@ -158,7 +162,7 @@ struct SweepArgs {
   const int* ch_seg;
   const long long* acc_in;  // nseg x K, old-row sums (complete at launch)
   long long* acc_out;       // kSweepRep x nseg x K, zero at launch
-  long long* obj_acc;       // kSweepRep x 2, zero at launch
+  long long* obj_acc;       // kRep x 2, zero at launch
   const int* cnt;           // nblk x B x kSweepRep contributions per acc_out entry
   long long rep_stride;
   float* O;
@@ -895,8 +899,9 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& a, const int wg, con
       // 2^-24 units to the accumulator's scale (a power of two >= 2^24:
       // exact for sums below 2^53)
       const double to_acc = a.scale_obj * 0x1p-24;
-      const unsigned long long r0 = fixed_add_wait(&a.obj_acc[2 * rep], (double)km4, to_acc);
-      const unsigned long long r1 = fixed_add_wait(&a.obj_acc[2 * rep + 1], (double)ent4, to_acc);
+      const int orep = wg % kRep;
+      const unsigned long long r0 = fixed_add_wait(&a.obj_acc[2 * orep], (double)km4, to_acc);
+      const unsigned long long r1 = fixed_add_wait(&a.obj_acc[2 * orep + 1], (double)ent4, to_acc);
       asm volatile("" ::"v"(r0 + r1));
     }
   }
@@ -915,7 +920,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& a, const int wg, con
   div = block_sum(div, red);
   if (tid == 0) {
     long long qk = 0, qe = 0;
-    for (int r = 0; r < kSweepRep; ++r) {
+    for (int r = 0; r < kRep; ++r) {
       qk += load_agent(&a.obj_acc[2 * r]);
       qe += load_agent(&a.obj_acc[2 * r + 1]);
     }
