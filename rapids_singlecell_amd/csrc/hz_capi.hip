@@ -766,11 +766,12 @@ static void bind_sweep_tab(SweepArgs& sw, const ChunkTab& t) {
 }
 
 // Shape rules of the block sweep: 16-row chunks, at most 256 clusters and
-// kSweepMaxBlocks blocks, one covariate, and R small enough for the sweep's
-// store offsets (n K 4 bytes <= kNoRow).
+// kSweepMaxBlocks blocks, one covariate, at least 4 PCs, and R and Z small
+// enough for the sweep's 32-bit buffer offsets (n K 4, n d 4 bytes <= kNoRow).
 static bool sweep_rules(const Layout& L) {
-  return L.chunk == 16 && L.K <= 256 && L.pd.nblk <= kSweepMaxBlocks && L.C == 1 &&
+  return L.chunk == 16 && L.K <= 256 && L.pd.nblk <= kSweepMaxBlocks && L.C == 1 && L.d >= 4 &&
          (unsigned long long)L.n * (unsigned long long)L.K * 4ull <= kNoRow &&
+         (unsigned long long)L.n * (unsigned long long)L.d * 4ull <= kNoRow &&
          g_tune[HZ_TUNE_SWEEP] == 0 && !sweep_broken();
 }
 

@@ -355,25 +355,33 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& a, const int wg, con
   // columns: in split-BF16 mode columns 32 s + 8 lg .. +7 of k-step s, in FP32
   // mode columns lg SM .. lg SM + SM - 1 (k-step s uses column lg SM + s).
   // Nothing is masked after the load, so the loads stay in flight until the
-  // MFMAs use them: a row past len reads row 0 (its results are never
+  // MFMAs use them: a row past len reads zeros (its results are never
   // stored) and a column c >= d reads the next row or zeros, which meet the
   // zero rows c >= d of the staged centroids.
   const __amdgpu_buffer_rsrc_t z_rsrc = __builtin_amdgcn_make_buffer_rsrc(
       const_cast<float*>(a.Z), (short)0, (int)((unsigned)a.n * (unsigned)d * 4u), 0x00020000);
+  // A load wholly past column d, or of a row past len, gets an offset past
+  // the end of Z: it returns zeros without touching memory.  The load that
+  // straddles column d reads up to 3 columns of the next row (they meet zero
+  // centroid rows); for the last row, the hardware range-checks each dword of
+  // a buffer load and returns zeros for those past the end of Z.  Offsets
+  // stay below 2^32 (the host checks n d 4 <= kNoRow).
   auto load_z = [&](int my_cell, int len, float (&za)[NZ]) __attribute__((always_inline)) {
     const int zc = __shfl(my_cell, lr, kWave);
-    const unsigned zoff = (unsigned)(lr < len ? zc : 0) * (unsigned)d * 4u;
+    const unsigned zoff = (lr < len) ? (unsigned)zc * (unsigned)d * 4u : kNoRow;
 #pragma unroll
     for (int q = 0; q < NZ / 4; ++q) {
       const int col = (PREC == kPrecF32) ? lg * SM + 4 * q : 32 * (q / 2) + 8 * lg + 4 * (q % 2);
+      const unsigned off = (col < d) ? zoff + 4u * (unsigned)col : 0x80000000u;
       // (the whole vector is bit-cast: element-wise casts of the loaded
       // vector let the optimiser shrink the load to its first dword)
       const floatx4 v = __builtin_bit_cast(
-          floatx4, __builtin_amdgcn_raw_buffer_load_b128(z_rsrc, (int)(zoff + 4u * (unsigned)col), 0, 0));
+          floatx4, __builtin_amdgcn_raw_buffer_load_b128(z_rsrc, (int)off, 0, 0));
 #pragma unroll
       for (int e = 0; e < 4; ++e) za[4 * q + e] = v[e];
     }
   };
+
   // MFMA chain of one tile; the B fragments (centroids in LDS) of the next
   // k-step are read before the MFMAs of the current one
   auto sim_mfma = [&](const float (&za)[NZ], floatx4 (&acc)[NT]) __attribute__((always_inline)) {
