@@ -399,17 +399,32 @@ __global__ __launch_bounds__(kBlock) void centroid_mfma_kernel(CentMArgs a) {
   floatx4 acc[MT];
 #pragma unroll
   for (int j = 0; j < MT; ++j) acc[j] = floatx4{0.f, 0.f, 0.f, 0.f};
+  // output tile (t, u) of accumulator slot j; false for slots past NTILE
+  // (their t, u are clamped to a valid tile).  With NU == 4, wave w owns
+  // column tile w and every row tile, so a k-step reads each row fragment once.
+  constexpr bool ROWS4 = (NU == 4);
+  auto tile_of = [&](int j, int& t, int& u) __attribute__((always_inline)) {
+    if constexpr (ROWS4) {
+      t = j;
+      u = wv;
+      return true;
+    } else {
+      const int i = wv + 4 * j;
+      t = i % NT;
+      u = min(i / NT, NU - 1);
+      return i < NTILE;
+    }
+  };
 
   __shared__ int s_cell[TR];
-  // load rows [p, p + rows) of `order` into the LDS tiles (zero beyond rows / K / d+1).
-  // All of a thread's loads are issued before any is used (addresses clamped
-  // in range, masks applied at the LDS store), so a tile costs one round trip.
-  auto load_tile = [&](int p, int rows) {
-    if (tid < TR) s_cell[tid] = a.order[p + min(tid, rows - 1)];
-    __syncthreads();
-    // 32-bit element offsets (the host keeps N*K and N*d below 2^31) let the
-    // loads use a scalar base plus a 32-bit vector offset
-    float rv[MR], zv[MZ];
+  // One tile = rows [p, p + rows) of `order` (rows <= TR, within a segment).
+  // Pipelined: while tile i is multiplied from LDS, tile i+1's R and Z loads
+  // are in flight into registers and tile i+2's cell ids into a register.
+  // 32-bit element offsets (the host keeps N*K and N*d below 2^31) let the
+  // loads use a scalar base plus a 32-bit vector offset; addresses are
+  // clamped in range and masks applied at the LDS store.
+  float rv[MR], zv[MZ];
+  auto issue_tile = [&]() {  // loads of the tile whose cells are in s_cell
 #pragma unroll
     for (int m = 0; m < MR; ++m) {
       const int e = tid + m * kBlock;
@@ -422,6 +437,8 @@ __global__ __launch_bounds__(kBlock) void centroid_mfma_kernel(CentMArgs a) {
       const int r = e / LWZ, cc = e % LWZ;
       zv[m] = a.Z[(unsigned)s_cell[r] * (unsigned)d + (unsigned)min(cc, d - 1)];
     }
+  };
+  auto stage_tile = [&](int rows) {  // registers -> LDS tiles (zero beyond rows / K / d+1)
 #pragma unroll
     for (int m = 0; m < MR; ++m) {
       const int e = tid + m * kBlock;
@@ -455,44 +472,99 @@ __global__ __launch_bounds__(kBlock) void centroid_mfma_kernel(CentMArgs a) {
   };
 
   // walk the segments of the chunk range; a segment's positions are
-  // contiguous, so its tiles are [p, min(seg_start[seg+1], range end))
+  // contiguous, so its tiles are [p, min(seg_start[seg+1], range end)).
+  // A tile cursor: segment, position, segment end (valid while seg >= 0).
+  struct Cur {
+    int seg, p, pend;
+  };
   const int range_end = (c1 > c0) ? a.ch_start[c1 - 1] + a.ch_len[c1 - 1] : 0;
-  int c = c0;
-  int seg = 0, p = 0, pend = 0;
-  if (c < c1) {
-    seg = a.ch_seg[c];
-    p = a.ch_start[c];
-    pend = min(a.seg_start[seg + 1], range_end);
-  }
-  while (c < c1) {
-    const int rows = min(TR, pend - p);
-    __syncthreads();  // previous tile's LDS reads are done
-    load_tile(p, rows);
+  auto seg_from = [&](int c) {  // cursor at chunk c's segment start (or none)
+    Cur u{-1, 0, 0};
+    if (c < c1) {
+      u.seg = a.ch_seg[c];
+      u.p = a.ch_start[c];
+      u.pend = min(a.seg_start[u.seg + 1], range_end);
+    }
+    return u;
+  };
+  auto advance = [&](const Cur& u) {  // the tile after u's tile
+    if (u.seg < 0) return u;
+    const int rows = min(TR, u.pend - u.p);
+    if (u.p + rows >= u.pend) return seg_from(a.seg_ch[u.seg + 1]);
+    return Cur{u.seg, u.p + rows, u.pend};
+  };
+  auto cell_of = [&](const Cur& u) {  // this thread's cell id of u's tile (tid < TR)
+    if (u.seg < 0) return 0;
+    const int rows = min(TR, u.pend - u.p);
+    return a.order[u.p + min(tid, rows - 1)];
+  };
+
+  Cur cur = seg_from(c0);
+  Cur nxt = advance(cur);
+  if (cur.seg >= 0) {
+    if (tid < TR) s_cell[tid] = cell_of(cur);
     __syncthreads();
-    const bool seg_done = p + rows >= pend;
-    // MFMA over the staged tile
+    issue_tile();
+  }
+  int cell_next = (tid < TR) ? cell_of(nxt) : 0;
+  while (cur.seg >= 0) {
+    const int seg = cur.seg;
+    const int rows = min(TR, cur.pend - cur.p);
+    __syncthreads();  // previous tile's LDS reads (and the issue's s_cell reads) are done
+    stage_tile(rows);
+    if (tid < TR) s_cell[tid] = cell_next;
+    __syncthreads();
+    const Cur nn = advance(nxt);
+    if (nxt.seg >= 0) issue_tile();  // next tile's loads, in flight during the MFMAs
+    cell_next = (tid < TR) ? cell_of(nn) : 0;
+    const bool seg_done = cur.p + rows >= cur.pend;
+    // MFMA over the staged tile.  No branch on the tile index (it would
+    // split the accumulators across control flow): slots past NTILE redo a
+    // valid tile into accumulators that are never written out.
     if (PREC == kPrecF32) {
-#pragma unroll 4
+#pragma unroll 2
       for (int q = 0; q < TR / 4; ++q) {
+        if constexpr (ROWS4) {
+          float av[NT];
 #pragma unroll
-        for (int j = 0; j < MT; ++j) {
-          const int i = wv + 4 * j;
-          if (i < NTILE) {
-            const int t = i % NT, u = i / NT;
-            const float av = Rs[(4 * q + lg) * SR + 16 * t + lr];
-            const float bv = Zs[(4 * q + lg) * SZ + 16 * u + lr];
-            acc[j] = mfma_f32(av, bv, acc[j]);
+          for (int t = 0; t < NT; ++t) av[t] = Rs[(4 * q + lg) * SR + 16 * t + lr];
+          const float bv = Zs[(4 * q + lg) * SZ + 16 * wv + lr];
+#pragma unroll
+          for (int t = 0; t < NT; ++t) acc[t] = mfma_f32(av[t], bv, acc[t]);
+        } else {
+          float av[MT], bv[MT];
+#pragma unroll
+          for (int j = 0; j < MT; ++j) {
+            int t, u;
+            tile_of(j, t, u);
+            av[j] = Rs[(4 * q + lg) * SR + 16 * t + lr];
+            bv[j] = Zs[(4 * q + lg) * SZ + 16 * u + lr];
           }
+#pragma unroll
+          for (int j = 0; j < MT; ++j) acc[j] = mfma_f32(av[j], bv[j], acc[j]);
         }
       }
     } else {
 #pragma unroll
       for (int q = 0; q < TR / 32; ++q) {
+        if constexpr (ROWS4) {
+          const int rb = (16 * wv + lr) * ST + 32 * q + 8 * lg;
+          const bf16x8 bh = *reinterpret_cast<const bf16x8*>(ZTh + rb);
+          const bf16x8 bl = *reinterpret_cast<const bf16x8*>(ZTl + rb);
 #pragma unroll
-        for (int j = 0; j < MT; ++j) {
-          const int i = wv + 4 * j;
-          if (i < NTILE) {
-            const int t = i % NT, u = i / NT;
+          for (int t = 0; t < NT; ++t) {
+            const int ra = (16 * t + lr) * ST + 32 * q + 8 * lg;
+            const bf16x8 ah = *reinterpret_cast<const bf16x8*>(RTh + ra);
+            const bf16x8 al = *reinterpret_cast<const bf16x8*>(RTl + ra);
+            acc[t] = mfma_bf16(ah, bh, acc[t]);
+            acc[t] = mfma_bf16(ah, bl, acc[t]);
+            acc[t] = mfma_bf16(al, bh, acc[t]);
+          }
+        } else {
+#pragma unroll
+          for (int j = 0; j < MT; ++j) {
+            int t, u;
+            tile_of(j, t, u);
             const int ra = (16 * t + lr) * ST + 32 * q + 8 * lg;
             const int rb = (16 * u + lr) * ST + 32 * q + 8 * lg;
             const bf16x8 ah = *reinterpret_cast<const bf16x8*>(RTh + ra);
@@ -510,9 +582,9 @@ __global__ __launch_bounds__(kBlock) void centroid_mfma_kernel(CentMArgs a) {
       // flush the ones column (tile u == ud, column jd) into acc_in
 #pragma unroll
       for (int j = 0; j < MT; ++j) {
-        const int i = wv + 4 * j;
-        if (i < NTILE && i / NT == ud && lr == jd) {
-          const int t = i % NT;
+        int t, u;
+        const bool ok = tile_of(j, t, u);
+        if (ok && u == ud && lr == jd) {
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int k = 16 * t + 4 * lg + r;
@@ -524,24 +596,15 @@ __global__ __launch_bounds__(kBlock) void centroid_mfma_kernel(CentMArgs a) {
         }
       }
     }
-    if (seg_done) {
-      c = a.seg_ch[seg + 1];  // first chunk after this segment
-      if (c < c1) {
-        seg = a.ch_seg[c];
-        p = a.ch_start[c];
-        pend = min(a.seg_start[seg + 1], range_end);
-      }
-    } else {
-      p += rows;
-    }
+    cur = nxt;
+    nxt = nn;
   }
   // write this workgroup's K x d partial
   float* out = a.Ypart + (long long)blockIdx.x * K * d;
 #pragma unroll
   for (int j = 0; j < MT; ++j) {
-    const int i = wv + 4 * j;
-    if (i < NTILE) {
-      const int t = i % NT, u = i / NT;
+    int t, u;
+    if (tile_of(j, t, u)) {
       const int cc = 16 * u + lr;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
