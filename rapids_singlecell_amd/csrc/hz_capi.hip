@@ -1496,11 +1496,34 @@ static int correction_impl(const hz_dims* dm, const T* X, const T* R, const T* O
   aa.n = L.n;
   aa.d = L.d;
   aa.K = L.K;
+  ProfScope ps(HZ_PROF_APPLY, s);
+  if constexpr (std::is_same<T, float>::value) {
+    if (L.mfma) {
+      // matrix cores: contiguous ranges of the 64-row chunks, two workgroups per CU
+      ApplyMArgs am{};
+      am.X = X;
+      am.R = R;
+      am.Wt = reinterpret_cast<const float*>(L.Wt);
+      am.Zhat = Zhat;
+      am.Znorm = Znorm;
+      am.order = L.st.order;
+      am.ch_start = L.st.ch_start;
+      am.ch_len = L.st.ch_len;
+      am.ch_seg = L.st.ch_seg;
+      am.nch = L.st.seg_ch + L.B;
+      const long long gm = std::max<long long>(1, std::min<long long>(2LL * device_cu_count(), L.sd.maxch));
+      am.cpw = (int)ceil_div(L.sd.maxch, gm);
+      am.n = L.n;
+      am.d = L.d;
+      am.K = L.K;
+      const int rc = launch_apply_mfma(L, am, (int)ceil_div(L.sd.maxch, (long long)am.cpw), s);
+      if (rc != HZ_ERR_UNSUPPORTED) return rc;
+    }
+  }
   const size_t lds = (size_t)L.K * L.Dp * sizeof(T) + (size_t)L.K * kZtStride * sizeof(T);
   HZ_RET_IF(lds > (size_t)device_lds_limit(), HZ_ERR_UNSUPPORTED,
             "apply kernel needs %zu bytes of LDS", lds);
   const int grid = (int)L.sd.maxch;
-  ProfScope ps(HZ_PROF_APPLY, s);
   switch (L.DT) {
     case 2: return launch_apply_dt<T, 2>(L, aa, grid, lds, s);
     case 4: return launch_apply_dt<T, 4>(L, aa, grid, lds, s);

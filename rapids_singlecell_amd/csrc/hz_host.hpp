@@ -93,6 +93,9 @@ int launch_cent_stream_f32(const Layout& L, const CentSArgs& a, int grid, hipStr
 int launch_cent_stream_bf16(const Layout& L, const CentSArgs& a, int grid, hipStream_t s);
 int launch_cent_mfma_f32(const Layout& L, const CentMArgs& a, int grid, hipStream_t s);
 int launch_cent_mfma_bf16(const Layout& L, const CentMArgs& a, int grid, hipStream_t s);
+// FP32 correction apply on matrix cores; HZ_ERR_UNSUPPORTED (nothing
+// launched) when its LDS does not fit
+int launch_apply_mfma(const Layout& L, const ApplyMArgs& a, int grid, hipStream_t s);
 void mark_sweep_broken();
 bool sweep_broken();
 

@@ -80,6 +80,34 @@ static int launch_cent_mfma_p(const Layout& L, const CentMArgs& a, int grid, hip
 
 
 #if HZ_UNIT_PREC == 0
+template <int CT, int DT>
+static int launch_am(const ApplyMArgs& a, int grid, hipStream_t s) {
+  constexpr size_t lds = apply_mfma_lds<CT, DT>();
+  if (lds > (size_t)device_lds_limit()) return HZ_ERR_UNSUPPORTED;
+  auto kern = apply_mfma_kernel<CT, DT>;
+  HZ_TRY(prepare_lds(reinterpret_cast<const void*>(kern), lds));
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), lds, s, a);
+  HZ_LAUNCHED("apply_mfma_kernel");
+  return HZ_OK;
+}
+
+template <int CT>
+static int launch_am_ct(const Layout& L, const ApplyMArgs& a, int grid, hipStream_t s) {
+  switch (L.DT) {
+    case 2: return launch_am<CT, 2>(a, grid, s);
+    case 4: return launch_am<CT, 4>(a, grid, s);
+    default: return launch_am<CT, 8>(a, grid, s);
+  }
+}
+
+int launch_apply_mfma(const Layout& L, const ApplyMArgs& a, int grid, hipStream_t s) {
+  switch (L.CT) {
+    case 4: return launch_am_ct<4>(L, a, grid, s);
+    case 8: return launch_am_ct<8>(L, a, grid, s);
+    default: return launch_am_ct<16>(L, a, grid, s);
+  }
+}
+
 int launch_cent_stream_f32(const Layout& L, const CentSArgs& a, int grid, hipStream_t s) {
   return launch_cs_p<kPrecF32>(L, a, grid, s);
 }

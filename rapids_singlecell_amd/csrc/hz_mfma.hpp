@@ -615,6 +615,153 @@ __global__ __launch_bounds__(kBlock) void centroid_mfma_kernel(CentMArgs a) {
   }
 }
 
+// ---------------------------------------------------------------- correction apply (FP32)
+
+struct ApplyMArgs {
+  const float* X;
+  const float* R;
+  const float* Wt;  // groups x K x d
+  float* Zhat;
+  float* Znorm;
+  const int* order;     // cells grouped by batch (group), 64-row chunks
+  const int* ch_start;
+  const int* ch_len;
+  const int* ch_seg;
+  const int* nch;  // number of chunks (device)
+  int cpw, n, d, K;
+};
+
+template <int CT, int DT>
+__host__ __device__ constexpr size_t apply_mfma_lds() {
+  // W_g [Kp][Dp + 16] and the R tile [64][Kp + 4], FP32
+  return (size_t)(16 * CT) * (16 * DT + 16) * 4 + (size_t)64 * (16 * CT + 4) * 4;
+}
+
+// Z_hat = X - R_g W_g and its L2-normalised rows, for a contiguous range of
+// 64-row chunks of the group-ordered cells (kernels_correction*.cuh apply
+// step).  The product runs on matrix cores in FP32 (v_mfma_f32_16x16x4):
+// wave w owns rows 16 w .. 16 w + 15 of a tile and every column tile, so a
+// row's squared norm is a sum over its lane's column tiles and 16 lanes.
+// W_g is staged in LDS once per group; while a tile is multiplied, the next
+// tile's R loads are in flight into registers and the tile after's cell ids
+// into a register; X of the current tile is loaded before the MFMAs.
+template <int CT, int DT>
+__global__ __launch_bounds__(kBlock) void apply_mfma_kernel(ApplyMArgs a) {
+  constexpr int TR = 64;
+  constexpr int Kp = 16 * CT, Dp = 16 * DT;
+  constexpr int WS = Dp + 16;  // W row stride (lane groups on distinct banks)
+  constexpr int RS = Kp + 4;   // R row stride
+  constexpr int LWR = (Kp <= 64) ? 64 : (Kp <= 128 ? 128 : 256);
+  constexpr int MR = TR * LWR / kBlock;
+  extern __shared__ __align__(16) unsigned char smem[];
+  float* Ws = reinterpret_cast<float*>(smem);
+  float* Rs = Ws + Kp * WS;
+  __shared__ int s_cell[2][TR];
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
+  const int lr = lane & 15, lg = lane >> 4;
+  const int d = a.d, K = a.K;
+  const int c0 = blockIdx.x * a.cpw;
+  const int c1 = min(*a.nch, c0 + a.cpw);
+  if (c0 >= c1) return;
+
+  int wg_group = -1;
+  auto stage_w = [&](int g) {  // W_g into LDS (zero beyond K and d); callers sync
+    const float* Wg = a.Wt + (long long)g * K * d;
+    for (int e = tid; e < Kp * Dp; e += kBlock) {
+      const int k = e / Dp, cc = e - k * Dp;
+      Ws[k * WS + cc] = (k < K && cc < d) ? Wg[(long long)k * d + cc] : 0.f;
+    }
+  };
+  float rv[MR];
+  auto cell_of = [&](int c) {  // this thread's cell of chunk c (tid < TR; row 0's past len)
+    if (c >= c1) return 0;
+    const int len = a.ch_len[c];
+    return a.order[a.ch_start[c] + min(tid, len - 1)];
+  };
+  auto issue_r = [&](int buf) {  // R rows of the tile whose cells are in s_cell[buf]
+#pragma unroll
+    for (int m = 0; m < MR; ++m) {
+      const int e = tid + m * kBlock;
+      const int r = e / LWR, k = e % LWR;
+      rv[m] = a.R[(unsigned)s_cell[buf][r] * (unsigned)K + (unsigned)min(k, K - 1)];
+    }
+  };
+  auto stage_r = [&](int rows) {
+#pragma unroll
+    for (int m = 0; m < MR; ++m) {
+      const int e = tid + m * kBlock;
+      const int r = e / LWR, k = e % LWR;
+      if (k < Kp) Rs[r * RS + k] = (r < rows && k < K) ? rv[m] : 0.f;
+    }
+  };
+
+  if (tid < TR) s_cell[0][tid] = cell_of(c0);
+  __syncthreads();
+  issue_r(0);
+  int cell_next = (tid < TR) ? cell_of(c0 + 1) : 0;
+  for (int c = c0; c < c1; ++c) {
+    const int buf = (c - c0) & 1;
+    const int rows = a.ch_len[c];
+    const int g = a.ch_seg[c];
+    __syncthreads();  // the previous tile's LDS reads and R issue are done
+    if (g != wg_group) {
+      stage_w(g);
+      wg_group = g;
+    }
+    stage_r(rows);
+    if (tid < TR) s_cell[buf ^ 1][tid] = cell_next;
+    __syncthreads();
+    if (c + 1 < c1) issue_r(buf ^ 1);
+    cell_next = (tid < TR) ? cell_of(c + 2) : 0;
+    // X of this wave's rows (lane: rows 16 w + 4 lg + r, columns 16 t + lr)
+    float xv[4][DT];
+    int row[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int rr = 16 * wv + 4 * lg + r;
+      row[r] = (rr < rows) ? s_cell[buf][rr] : -1;
+#pragma unroll
+      for (int t = 0; t < DT; ++t)
+        xv[r][t] = a.X[(unsigned)max(row[r], 0) * (unsigned)d + (unsigned)min(16 * t + lr, d - 1)];
+    }
+    floatx4 acc[DT];
+#pragma unroll
+    for (int t = 0; t < DT; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
+    for (int q = 0; q < Kp / 4; ++q) {
+      const float av = Rs[(16 * wv + lr) * RS + 4 * q + lg];
+      float bv[DT];
+#pragma unroll
+      for (int t = 0; t < DT; ++t) bv[t] = Ws[(4 * q + lg) * WS + 16 * t + lr];
+#pragma unroll
+      for (int t = 0; t < DT; ++t) acc[t] = mfma_f32(av, bv[t], acc[t]);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      float z[DT];
+      float ss = 0.f;
+#pragma unroll
+      for (int t = 0; t < DT; ++t) {
+        z[t] = (16 * t + lr < d) ? xv[r][t] - acc[t][r] : 0.f;
+        ss = __builtin_fmaf(z[t], z[t], ss);
+      }
+      ss = sum16(ss);
+      float inv = 1.f / sqrtf(ss);
+      if (!(inv <= 1e12f)) inv = 1e12f;
+      if (row[r] >= 0) {
+#pragma unroll
+        for (int t = 0; t < DT; ++t) {
+          const int cc = 16 * t + lr;
+          if (cc < d) {
+            a.Zhat[(long long)row[r] * d + cc] = z[t];
+            a.Znorm[(long long)row[r] * d + cc] = z[t] * inv;
+          }
+        }
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------- streaming centroid pass
 
 struct CentSArgs {
