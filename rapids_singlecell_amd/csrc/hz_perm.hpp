@@ -23,8 +23,11 @@ namespace hz {
 constexpr int kScanThreads = 256;
 constexpr int kScanPer = 8;  // consecutive elements per thread and tile
 constexpr int kGroupTile = 1024;  // positions per tile in the grouping passes
-constexpr int kBoundCap = 1024;   // keys kept per boundary bucket (buckets hold ~32-64)
-constexpr int kMemberTile = 2048; // cells per tile in the membership grouping passes
+// keys kept per boundary bucket: the membership buckets hold ~512-1024 keys
+// (coarse, LDS-counted) or ~32-512 (fine, above 2^24 cells)
+constexpr int kBoundCap = 2048;
+constexpr int kMemberTile = 8192;  // cells per tile in the membership grouping passes
+constexpr int kMemberLdsBits = 14; // largest coarse histogram counted in LDS (64 KiB)
 
 __device__ __forceinline__ uint32_t pcg_key(uint32_t i, uint32_t seed) {
   // PCG-RXS-M-XS 32-bit output permutation applied to (i ^ seed).
@@ -175,6 +178,36 @@ static __global__ void perm_keys_kernel(const int* __restrict__ done, int n, uin
     uint32_t k = pcg_key((uint32_t)i, seed);
     keys[i] = k;
     atomicAdd(&bhist[k >> shift], 1);
+  }
+}
+
+// The same keys, with a histogram of their top `32 - shift` bits counted in
+// LDS (nbk ints) over `per_wg` consecutive cells per workgroup and added to
+// the global histogram once per bucket.  The global adds number
+// ceil(n / per_wg) * nbk per iteration instead of one per key.
+static __global__ __launch_bounds__(256) void perm_keys_lds_kernel(int n, uint32_t seed, int shift,
+                                                                   int nbk, int per_wg,
+                                                                   uint32_t* __restrict__ keys,
+                                                                   int* __restrict__ bhist,
+                                                                   PermStride st) {
+  extern __shared__ int h[];
+  const int y = blockIdx.y;
+  seed += (uint32_t)y;
+  keys += st.n * y;
+  bhist += st.bk * y;
+  for (int b = threadIdx.x; b < nbk; b += blockDim.x) h[b] = 0;
+  __syncthreads();
+  const long long i0 = (long long)blockIdx.x * per_wg;
+  const long long i1 = min((long long)n, i0 + per_wg);
+  for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    const uint32_t k = pcg_key((uint32_t)i, seed);
+    keys[i] = k;
+    atomicAdd(&h[k >> shift], 1);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < nbk; b += blockDim.x) {
+    const int c = h[b];
+    if (c) atomicAdd(&bhist[b], c);
   }
 }
 
@@ -528,22 +561,24 @@ static __global__ __launch_bounds__(256) void bound_select_kernel(
   }
 }
 
-// block of a key: number of boundary keys thr[1..nblk-1] (increasing) <= key
-__device__ __forceinline__ int block_of(const uint32_t* __restrict__ sthr, int nblk, uint32_t k) {
-  int lo = 0, hi = nblk - 1;  // answer in [0, nblk-1]
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (sthr[mid] <= k) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
+// Block of a key: the number of boundary keys thr[1..nblk-1] (increasing)
+// that are <= k.  The keys are uniform over 2^32, so block j holds about the
+// keys [j, j+1) * bs * 2^32 / n: start from that guess (kscale =
+// n / (bs 2^32)) and step to the exact block, which is rarely needed
+// (only keys within a few block-size standard deviations of a boundary).
+__device__ __forceinline__ int block_of(const uint32_t* __restrict__ sthr, int nblk, float kscale,
+                                        uint32_t k) {
+  int j = min(nblk - 1, (int)((float)k * kscale));
+  while (j > 0 && k < sthr[j]) --j;
+  while (j < nblk - 1 && k >= sthr[j + 1]) ++j;
+  return j;
 }
 
 // (segment, tile) histogram of the cells, tiles of T2 consecutive cells.
 // LDS: nseg counters, then nblk thresholds.
 static __global__ __launch_bounds__(256) void member_count_kernel(
     const uint32_t* __restrict__ keys, const uint32_t* __restrict__ thr,
-    const int* __restrict__ cats, int n, int nblk, int nb, int tiles, int T2,
+    const int* __restrict__ cats, int n, int nblk, int nb, int tiles, int T2, float kscale,
     int* __restrict__ thist, PermStride st) {
   extern __shared__ int lds[];
   const int y = blockIdx.y;
@@ -560,17 +595,20 @@ static __global__ __launch_bounds__(256) void member_count_kernel(
   const long long c0 = (long long)tile * T2;
   const long long c1 = min((long long)n, c0 + T2);
   for (long long i = c0 + threadIdx.x; i < c1; i += blockDim.x)
-    atomicAdd(&hist[block_of(sthr, nblk, keys[i]) * nb + cats[i]], 1);
+    atomicAdd(&hist[block_of(sthr, nblk, kscale, keys[i]) * nb + cats[i]], 1);
   __syncthreads();
   for (int e = threadIdx.x; e < nseg; e += blockDim.x) thist[(long long)e * tiles + tile] = hist[e];
 }
 
 // Stable placement in cell order: wave w takes quarter w of the tile; its
 // first slot in each segment is the tile's segment offset plus the counts of
-// the waves before it.  LDS: 4 x nseg slots, then nblk thresholds.
+// the waves before it.  Per 64 cells, the lanes holding the same segment are
+// found with one ballot per segment bit, so a lane's rank among them (lanes
+// below it) and their count take a few bit counts whatever the number of
+// distinct segments.  LDS: 4 x nseg slots, then nblk thresholds.
 static __global__ __launch_bounds__(256) void member_place_kernel(
     const uint32_t* __restrict__ keys, const uint32_t* __restrict__ thr,
-    const int* __restrict__ cats, int n, int nblk, int nb, int tiles, int T2,
+    const int* __restrict__ cats, int n, int nblk, int nb, int tiles, int T2, float kscale,
     const int* __restrict__ tbase, int* __restrict__ order, PermStride st) {
   extern __shared__ int lds[];
   const int y = blockIdx.y;
@@ -579,6 +617,7 @@ static __global__ __launch_bounds__(256) void member_place_kernel(
   tbase += st.th * y;
   order += st.n * y;
   const int nseg = nblk * nb;
+  const int sbits = 32 - __builtin_clz((unsigned)max(nseg - 1, 1));  // bits of a segment id
   int* slot = lds;  // [wave][segment]
   uint32_t* sthr = reinterpret_cast<uint32_t*>(lds + 4 * nseg);
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -591,7 +630,7 @@ static __global__ __launch_bounds__(256) void member_place_kernel(
   const long long q = (c1 - c0 + 3) / 4;
   const long long w0 = c0 + w * q, w1 = min(c1, w0 + q);
   for (long long i = w0 + lane; i < w1; i += 64)
-    atomicAdd(&slot[w * nseg + block_of(sthr, nblk, keys[i]) * nb + cats[i]], 1);
+    atomicAdd(&slot[w * nseg + block_of(sthr, nblk, kscale, keys[i]) * nb + cats[i]], 1);
   __syncthreads();
   for (int e = tid; e < nseg; e += blockDim.x) {
     int run = tbase[(long long)e * tiles + tile];
@@ -602,21 +641,26 @@ static __global__ __launch_bounds__(256) void member_place_kernel(
     }
   }
   __syncthreads();
-  const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   int* myslot = slot + w * nseg;
   for (long long base = w0; base < w1; base += 64) {
     const long long i = base + lane;
     const bool valid = i < w1;
-    const int sg = valid ? block_of(sthr, nblk, keys[i]) * nb + cats[i] : -1;
-    unsigned long long active = __ballot(valid);
-    while (active) {
-      const int leader = __builtin_ctzll(active);
-      const int s0 = __builtin_amdgcn_readlane(sg, leader);
-      const unsigned long long m = __ballot(sg == s0) & active;
-      const int start = myslot[s0];
-      if (sg == s0) order[start + __popcll(m & lt)] = (int)i;
-      if (lane == leader) myslot[s0] = start + __popcll(m);
-      active &= ~m;
+    const int sg = valid ? block_of(sthr, nblk, kscale, keys[i]) * nb + cats[i] : 0;
+    // lanes with this lane's segment: agree on every segment bit
+    unsigned long long match = __ballot(valid);
+    for (int b = 0; b < sbits; ++b) {
+      const bool bit = (sg >> b) & 1;
+      const unsigned long long m = __ballot(bit);
+      match &= bit ? m : ~m;
+    }
+    const int rank = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(match >> 32),
+                                                    __builtin_amdgcn_mbcnt_lo((unsigned)match, 0u));
+    // one read of every valid lane's slot, then the group's first lane moves
+    // it past the group (LDS operations of a wave complete in order)
+    const int start = valid ? myslot[sg] : 0;
+    if (valid) {
+      order[start + rank] = (int)i;
+      if (rank == 0) myslot[sg] = start + __popcll(match);
     }
   }
 }
@@ -662,15 +706,33 @@ static int launch_key_order(const PermDims& p, uint32_t seed, PermBufs& b, const
 // iterations (seeds seed .. seed+I-1) in one set of launches.
 static int launch_member_order(const PermDims& p, uint32_t seed, const int* cats, PermBufs& b,
                                ChunkTab& t, int* nch_tmp, int* err, hipStream_t s, int I) {
-  const int shift = 32 - p.nbk_bits;
-  const int nbk = 1 << p.nbk_bits;
+  // Membership needs only the bucket of each block boundary, so its buckets
+  // are coarse (~512-1024 keys each) and counted in LDS while their histogram
+  // fits kMemberLdsBits; above that (more than ~2^24 cells) the fine buckets
+  // of the full ranking are counted with global atomics.
+  const int lg = bit_length((unsigned long long)std::max(p.n - 1, 1));
+  const int cbits = std::min(p.nbk_bits, std::max(1, lg - 10));
+  const bool lds_hist = cbits <= kMemberLdsBits;
+  const int bits = lds_hist ? cbits : p.nbk_bits;
+  const int shift = 32 - bits;
+  const int nbk = 1 << bits;
   const PermStride st = perm_stride(p);
   const long long nseg = (long long)p.nblk * p.nb;
   HZ_TRY(fill<int>(b.bhist, st.bk * I, 0, s));
   const int g = launch_grid_1d(p.n, I > 1 ? 2 : 8);
-  hipLaunchKernelGGL(perm_keys_kernel, dim3(g, I), dim3(256), 0, s, nullptr, p.n, seed, shift,
-                     b.keys, b.bhist, st);
-  HZ_LAUNCHED("perm_keys");
+  if (lds_hist) {
+    // at least 16 keys per bucket and workgroup on average: the global adds
+    // stay below n / 16 + nbk per iteration
+    const int per_wg = std::max(32768, 16 * nbk);
+    hipLaunchKernelGGL(perm_keys_lds_kernel, dim3((int)ceil_div(p.n, per_wg), I), dim3(256),
+                       (size_t)nbk * sizeof(int), s, p.n, seed, shift, nbk, per_wg, b.keys,
+                       b.bhist, st);
+    HZ_LAUNCHED("perm_keys_lds");
+  } else {
+    hipLaunchKernelGGL(perm_keys_kernel, dim3(g, I), dim3(256), 0, s, nullptr, p.n, seed, shift,
+                       b.keys, b.bhist, st);
+    HZ_LAUNCHED("perm_keys");
+  }
   hipLaunchKernelGGL(perm_bucket_scan_kernel, dim3(1, I), dim3(kScanThreads), 0, s, nullptr,
                      b.bhist, b.bstart, b.bcur, nbk, st);
   HZ_LAUNCHED("perm_bucket_scan");
@@ -689,9 +751,11 @@ static int launch_member_order(const PermDims& p, uint32_t seed, const int* cats
   }
   const int T2 = kMemberTile;
   const int tiles = (int)ceil_div(p.n, T2);
+  // block-of-key guess: n / (bs 2^32)
+  const float kscale = (float)((double)p.n / ((double)p.bs * 4294967296.0));
   hipLaunchKernelGGL(member_count_kernel, dim3(tiles, I), dim3(256),
                      (size_t)(nseg + p.nblk) * sizeof(int), s, b.keys, b.thr, cats, p.n, p.nblk,
-                     p.nb, tiles, T2, b.thist, st);
+                     p.nb, tiles, T2, kscale, b.thist, st);
   HZ_LAUNCHED("member_count");
   hipLaunchKernelGGL(chunk_table_kernel, dim3(1, I), dim3(kScanThreads), 0, s, nullptr, b.thist,
                      b.tbase, nseg, tiles, p.n, p.chunk, t.seg_start, t.seg_ch, t.ch_start,
@@ -703,7 +767,7 @@ static int launch_member_order(const PermDims& p, uint32_t seed, const int* cats
   HZ_LAUNCHED("chunk_fill");
   hipLaunchKernelGGL(member_place_kernel, dim3(tiles, I), dim3(256),
                      (size_t)(4 * nseg + p.nblk) * sizeof(int), s, b.keys, b.thr, cats, p.n,
-                     p.nblk, p.nb, tiles, T2, b.tbase, t.order, st);
+                     p.nblk, p.nb, tiles, T2, kscale, b.tbase, t.order, st);
   HZ_LAUNCHED("member_place");
   return HZ_OK;
 }

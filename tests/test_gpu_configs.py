@@ -160,3 +160,64 @@ def test_c2_split_bf16_fixed_iters(device, c2):
     print(f"\nC2 split-BF16 2x20 iters: rel err Z_corr vs FP64 oracle = {e:.3e}; "
           f"R = {eR:.3e}", flush=True)
     assert e < 1e-3
+
+
+def _c3_job(X, codes, B, K, precision):
+    from rapids_singlecell_amd.preprocessing._harmony._job import HarmonyJob
+
+    dt = X.dtype
+    N_b = np.bincount(codes, minlength=B).astype(dt)
+    return HarmonyJob(torch.as_tensor(X, device="cuda"), codes, B, n_clusters=K,
+                      theta_np=np.full(B, 2.0, dt), N_b_np=N_b, sigma=0.1,
+                      block_proportion=0.05, max_iter_clustering=20,
+                      tol_clustering=NEG_INF, ridge_lambda=1.0, stabilized_penalty=False,
+                      dynamic_lambda=False, correction_method="batched", random_state=0,
+                      precision=precision)
+
+
+def test_c3_full_size_invariants(device):
+    """C3 at full size (1.3M cells x 50 PCs, K = 100, 16 uniform one-hot
+    batches), where the oracle would take minutes per iteration, through
+    properties that hold at any size.  After 20 clustering iterations on the
+    block-sweep path: every row of R sums to 1; O_b equals the column sums of
+    R over batch b's cells and E = Pr_b x colsum R (the sweep carries O and E
+    block by block through its fixed-point exchange, so a lost or misplaced
+    block sum would show as a percent-level error).  One Harmony iteration in FP32 and in split-BF16 lands
+    within 1e-4 and 1e-3 of the FP64 path, which the C1 and C2 tests pin to
+    the oracle."""
+    n, d, K, B = 1_300_000, 50, 100, 16
+    X, obs = synthetic_pcs(n, d, 50, B, seed=0, shift_scale=0.5)
+    codes = np.asarray(obs["batch"].cat.codes.values, dtype=np.int32)
+    Z = ho.l2_normalize(X.astype(np.float64))
+    Y0 = Z[np.random.default_rng(7).choice(n, K, replace=False)]
+    cats = torch.as_tensor(codes.astype(np.int64), device="cuda")
+    Pr = torch.as_tensor(np.bincount(codes, minlength=B) / n, device="cuda")
+    out = {}
+    for name, Xs, prec in (("fp64", X.astype(np.float64), 0), ("fp32", X, 0),
+                           ("split_bf16", X, 1)):
+        print(f"\nC3 {name}: 20 clustering iterations", flush=True)
+        job = _c3_job(Xs, codes, B, K, prec)
+        job.init(torch.as_tensor(Y0.astype(Xs.dtype)))
+        assert job.clustering(0) == 20
+        torch.cuda.synchronize()
+        R = job.R.double()
+        row_err = float((R.sum(1) - 1).abs().max())
+        colsum = R.sum(0)
+        O_ref = torch.zeros(B, K, dtype=torch.float64, device="cuda").index_add_(0, cats, R)
+        eO = float((job.O.double() - O_ref).abs().max() / O_ref.abs().max())
+        eE = float((job.E.double() - Pr[:, None] * colsum[None, :]).abs().max()
+                   / colsum.abs().max())
+        print(f"C3 {name}: max |row sum - 1| = {row_err:.2e}, O rel {eO:.2e}, E rel {eE:.2e}",
+              flush=True)
+        # O and E are float accumulators updated twice per block (20 blocks x
+        # 20 iterations, as the reference updates them in the storage type):
+        # up to 400 roundings of half an ulp, 2.4e-5 relative in FP32
+        tol = 1e-9 if name == "fp64" else 1e-4
+        assert row_err < tol * 1e-1 and eO < tol and eE < tol
+        job.correction()
+        out[name] = job.Z_hat.cpu().numpy()
+        del job, R, O_ref
+    e32, ebf = rel(out["fp32"], out["fp64"]), rel(out["split_bf16"], out["fp64"])
+    print(f"C3 one Harmony iteration: Z_hat rel err vs FP64 path: FP32 {e32:.2e}, "
+          f"split-BF16 {ebf:.2e}", flush=True)
+    assert e32 < 1e-4 and ebf < 1e-3

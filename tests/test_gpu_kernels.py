@@ -234,6 +234,19 @@ def test_block_membership_bit_exact(device, n, B, seed):
     _check_membership(n, bs, cats, seed, blk, order)
 
 
+@pytest.mark.parametrize("n", [9_000_000, 17_000_000])
+def test_block_membership_large(device, n):
+    """Past 2^23 cells the boundary buckets' histogram takes the whole 64 KiB
+    of LDS (9M cells); past 2^24 (17M) the fine buckets are counted with
+    global atomics instead.  Both give the reference's membership."""
+    B = 6
+    cats = np.random.default_rng(n % 1000).integers(0, B, n).astype(np.int32)
+    bs = int(n * 0.05)
+    print(f"\nmembership n={n}", flush=True)
+    blk, order = _membership(device, n, bs, cats, 12345, 2)
+    _check_membership(n, bs, cats, 12345, blk, order)
+
+
 def test_block_membership_key_order_fallback(device):
     """100 blocks x 41 batches = 4100 segments do not fit the membership
     kernels' LDS: the full key ranking is used instead; same membership."""
