@@ -982,19 +982,25 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
     }
     if (it % L.nord == 0) HZ_TRY(build_orders(it));
     const ChunkTab t = tab_of(L.ct, L.pd, it % L.nord);
-    // side stream: the old-row sums of this iteration's blocks, once R of the
-    // previous iteration is final
-    if (it >= 1) HZ_HIP(hipStreamWaitEvent(s2, ss->freed[(it - 1) & 1], 0));
     if (use_sw) {
+      // caller's stream, once the previous sweep has finished with R and the
+      // old-row sums: clear those sums and count each block sum's
+      // contributors; then the side stream sums this iteration's old rows
+      // beside the centroid pass (only that kernel waits on the side stream)
+      {
+        ProfScope ps(HZ_PROF_ORDER, s);
+        HZ_TRY(fill<long long>(L.acc_in, L.nacc * L.K, 0, s));
+        hipLaunchKernelGGL(sweep_count_kernel, dim3(L.pd.nblk), dim3(kBlock),
+                           (size_t)L.B * kSweepRep * sizeof(int), s, &L.ctrl->done, t.seg_ch,
+                           t.ch_seg, L.swcnt, L.B, sweep_grid);
+        HZ_LAUNCHED("sweep_count_kernel");
+      }
+      HZ_HIP(hipEventRecord(ss->freed[it & 1], s));
+      HZ_HIP(hipStreamWaitEvent(s2, ss->freed[it & 1], 0));
       ia.order = t.order;
       ia.seg_start = t.seg_start;
       ProfScope ps(HZ_PROF_ORDER, s2);
-      HZ_TRY(fill<long long>(L.acc_in, L.nacc * L.K, 0, s2));
       HZ_TRY(launch_insum(L, ia, s2));
-      hipLaunchKernelGGL(sweep_count_kernel, dim3(L.pd.nblk), dim3(kBlock),
-                         (size_t)L.B * kSweepRep * sizeof(int), s2, &L.ctrl->done, t.seg_ch,
-                         t.ch_seg, L.swcnt, L.B, sweep_grid);
-      HZ_LAUNCHED("sweep_count_kernel");
     }
     HZ_HIP(hipEventRecord(ss->ready[it & 1], s2));
     bind_tab(t);
@@ -1062,7 +1068,6 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
         }
       }
     }
-    HZ_HIP(hipEventRecord(ss->freed[it & 1], s));
   }
   // the caller's stream waits for the side stream's last table
   HZ_HIP(hipEventRecord(ss->done, s2));
@@ -1286,9 +1291,22 @@ static int batched_impl(hz_job* jobs, int n, void* bws, size_t bws_bytes, hipStr
       HZ_HIP(hipEventRecord(ss->tabs, s));
       HZ_HIP(hipStreamWaitEvent(s2, ss->tabs, 0));
     }
-    // side stream: every job's old-row sums and contribution counts, once
-    // the previous sweep has finished with R
-    if (it >= 1) HZ_HIP(hipStreamWaitEvent(s2, ss->freed[(it - 1) & 1], 0));
+    // caller's stream, once the previous sweep has finished with R and the
+    // old-row sums: every job's sums cleared and contributors counted; then
+    // the side stream sums every job's old rows beside the centroid passes
+    for (int j = 0; j < n; ++j) {
+      BatchJob& b = J[j];
+      Layout& L = b.L;
+      const ChunkTab t = tab_of(L.ct, L.pd, it % L.nord);
+      ProfScope ps(HZ_PROF_ORDER, s);
+      HZ_TRY(fill<long long>(L.acc_in, L.nacc * L.K, 0, s));
+      hipLaunchKernelGGL(sweep_count_kernel, dim3(L.pd.nblk), dim3(kBlock),
+                         (size_t)L.B * kSweepRep * sizeof(int), s, &L.ctrl->done, t.seg_ch,
+                         t.ch_seg, L.swcnt, L.B, b.G);
+      HZ_LAUNCHED("sweep_count_kernel");
+    }
+    HZ_HIP(hipEventRecord(ss->freed[it & 1], s));
+    HZ_HIP(hipStreamWaitEvent(s2, ss->freed[it & 1], 0));
     for (int j = 0; j < n; ++j) {
       BatchJob& b = J[j];
       Layout& L = b.L;
@@ -1296,12 +1314,7 @@ static int batched_impl(hz_job* jobs, int n, void* bws, size_t bws_bytes, hipStr
       b.ia.order = t.order;
       b.ia.seg_start = t.seg_start;
       ProfScope ps(HZ_PROF_ORDER, s2);
-      HZ_TRY(fill<long long>(L.acc_in, L.nacc * L.K, 0, s2));
       HZ_TRY(launch_insum(L, b.ia, s2));
-      hipLaunchKernelGGL(sweep_count_kernel, dim3(L.pd.nblk), dim3(kBlock),
-                         (size_t)L.B * kSweepRep * sizeof(int), s2, &L.ctrl->done, t.seg_ch,
-                         t.ch_seg, L.swcnt, L.B, b.G);
-      HZ_LAUNCHED("sweep_count_kernel");
     }
     HZ_HIP(hipEventRecord(ss->ready[it & 1], s2));
     // caller's stream: every job's centroids (natural cell order)
@@ -1323,7 +1336,6 @@ static int batched_impl(hz_job* jobs, int n, void* bws, size_t bws_bytes, hipStr
       const SweepArgs* a = d_args + (size_t)it * n;
       HZ_TRY(chained_sweep(s, [&]() { return sweep_multi(J[0].L, lds, a, d_off, n, G, nullptr, s); }));
     }
-    HZ_HIP(hipEventRecord(ss->freed[it & 1], s));
   }
   HZ_HIP(hipEventRecord(ss->done, s2));
   HZ_HIP(hipStreamWaitEvent(s, ss->done, 0));

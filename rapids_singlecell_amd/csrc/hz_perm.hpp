@@ -506,23 +506,79 @@ static __global__ void bound_mark_kernel(const int* __restrict__ bstart, int* __
   for (int j = threadIdx.x; j < nblk; j += blockDim.x) bcnt[j] = 0;
 }
 
-// keys of the marked buckets, appended to the list of their first boundary
-static __global__ void bound_collect_kernel(const uint32_t* __restrict__ keys,
-                                     const int* __restrict__ bmark, int* __restrict__ bcnt,
-                                     uint32_t* __restrict__ blist, int n, int shift,
-                                     PermStride st) {
+// Keys of the marked buckets, appended to the list of their first boundary.
+// Each workgroup gathers the marked keys of its `slice` cells in LDS (with
+// its rank among the workgroup's keys of the same list), reserves room in
+// every list it touches with one global add, then copies: a list receives
+// one add per workgroup instead of one per key.  The bucket marks are copied
+// to LDS when there are at most kCollectLdsMarks buckets, and each thread
+// keeps kCollectUnroll key loads in flight.  More than kCollectCap marked
+// keys in a slice (the host sizes slices for about a quarter of that) sets
+// *err; the caller reports the order as failed.
+constexpr int kCollectCap = 2048;
+constexpr int kCollectLdsMarks = 16384;
+constexpr int kCollectUnroll = 8;
+static __global__ __launch_bounds__(256) void bound_collect_kernel(
+    const uint32_t* __restrict__ keys, const int* __restrict__ bmark, int* __restrict__ bcnt,
+    uint32_t* __restrict__ blist, int n, int shift, int nbk, int nblk, int slice,
+    int* __restrict__ err, PermStride st) {
+  __shared__ uint32_t lk[kCollectCap];
+  __shared__ int lm[kCollectCap];   // list (first boundary) of each gathered key
+  __shared__ int lsl[kCollectCap];  // its rank among this workgroup's keys of that list
+  __shared__ unsigned short lmark[kCollectLdsMarks];
+  __shared__ int ltot;
+  extern __shared__ int lc[];  // nblk counts, then nblk reserved bases
   const int y = blockIdx.y;
   keys += st.n * y;
   bmark += st.bk * y;
   bcnt += st.nb * y;
   blist += st.nb * kBoundCap * y;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const uint32_t k = keys[i];
-    const int m = bmark[k >> shift];
-    if (m) {
-      const int slot = atomicAdd(&bcnt[m], 1);
-      if (slot < kBoundCap) blist[(long long)m * kBoundCap + slot] = k;
+  const int tid = threadIdx.x;
+  const bool lds_marks = nbk <= kCollectLdsMarks;  // (marks are below nblk <= 65535)
+  if (lds_marks)
+    for (int b = tid; b < nbk; b += blockDim.x) lmark[b] = (unsigned short)bmark[b];
+  for (int m = tid; m < nblk; m += blockDim.x) lc[m] = 0;
+  if (tid == 0) ltot = 0;
+  __syncthreads();
+  const long long i0 = (long long)blockIdx.x * slice;
+  const long long i1 = min((long long)n, i0 + slice);
+  for (long long ib = i0; ib < i1; ib += (long long)blockDim.x * kCollectUnroll) {
+    uint32_t kv[kCollectUnroll];
+#pragma unroll
+    for (int u = 0; u < kCollectUnroll; ++u) {
+      const long long i = ib + (long long)blockDim.x * u + tid;
+      kv[u] = i < i1 ? keys[i] : 0u;
     }
+#pragma unroll
+    for (int u = 0; u < kCollectUnroll; ++u) {
+      const long long i = ib + (long long)blockDim.x * u + tid;
+      const uint32_t k = kv[u];
+      const int m = i >= i1 ? 0 : (lds_marks ? (int)lmark[k >> shift] : bmark[k >> shift]);
+      if (m) {
+        const int e = atomicAdd(&ltot, 1);
+        if (e < kCollectCap) {
+          lk[e] = k;
+          lm[e] = m;
+          lsl[e] = atomicAdd(&lc[m], 1);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const int tot = ltot;
+  if (tot > kCollectCap) {
+    if (tid == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return;
+  }
+  int* lbase = lc + nblk;
+  for (int m = tid; m < nblk; m += blockDim.x)
+    if (lc[m]) lbase[m] = atomicAdd(&bcnt[m], lc[m]);
+  __syncthreads();
+  for (int e = tid; e < tot; e += blockDim.x) {
+    const int m = lm[e];
+    const int slot = lbase[m] + lsl[e];
+    // a list past kBoundCap keeps its count (bound_select reports it)
+    if (slot < kBoundCap) blist[(long long)m * kBoundCap + slot] = lk[e];
   }
 }
 
@@ -574,11 +630,11 @@ __device__ __forceinline__ int block_of(const uint32_t* __restrict__ sthr, int n
   return j;
 }
 
-// (segment, tile) histogram of the cells, tiles of T2 consecutive cells.
+// (segment, tile) histogram of the cells, tiles of kMemberTile consecutive cells.
 // LDS: nseg counters, then nblk thresholds.
 static __global__ __launch_bounds__(256) void member_count_kernel(
     const uint32_t* __restrict__ keys, const uint32_t* __restrict__ thr,
-    const int* __restrict__ cats, int n, int nblk, int nb, int tiles, int T2, float kscale,
+    const int* __restrict__ cats, int n, int nblk, int nb, int tiles, float kscale,
     int* __restrict__ thist, PermStride st) {
   extern __shared__ int lds[];
   const int y = blockIdx.y;
@@ -592,10 +648,21 @@ static __global__ __launch_bounds__(256) void member_count_kernel(
   for (int e = threadIdx.x; e < nblk; e += blockDim.x) sthr[e] = thr[e];
   __syncthreads();
   const int tile = blockIdx.x;
-  const long long c0 = (long long)tile * T2;
-  const long long c1 = min((long long)n, c0 + T2);
-  for (long long i = c0 + threadIdx.x; i < c1; i += blockDim.x)
-    atomicAdd(&hist[block_of(sthr, nblk, kscale, keys[i]) * nb + cats[i]], 1);
+  const long long c0 = (long long)tile * kMemberTile;
+  const long long c1 = min((long long)n, c0 + kMemberTile);
+  // all of the thread's loads first, then the counts
+  constexpr int kPer = kMemberTile / 256;
+  uint32_t kv[kPer];
+  int cv[kPer];
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const long long i = c0 + 256 * u + threadIdx.x;
+    kv[u] = i < c1 ? keys[i] : 0u;
+    cv[u] = i < c1 ? cats[i] : -1;
+  }
+#pragma unroll
+  for (int u = 0; u < kPer; ++u)
+    if (cv[u] >= 0) atomicAdd(&hist[block_of(sthr, nblk, kscale, kv[u]) * nb + cv[u]], 1);
   __syncthreads();
   for (int e = threadIdx.x; e < nseg; e += blockDim.x) thist[(long long)e * tiles + tile] = hist[e];
 }
@@ -608,7 +675,7 @@ static __global__ __launch_bounds__(256) void member_count_kernel(
 // distinct segments.  LDS: 4 x nseg slots, then nblk thresholds.
 static __global__ __launch_bounds__(256) void member_place_kernel(
     const uint32_t* __restrict__ keys, const uint32_t* __restrict__ thr,
-    const int* __restrict__ cats, int n, int nblk, int nb, int tiles, int T2, float kscale,
+    const int* __restrict__ cats, int n, int nblk, int nb, int tiles, float kscale,
     const int* __restrict__ tbase, int* __restrict__ order, PermStride st) {
   extern __shared__ int lds[];
   const int y = blockIdx.y;
@@ -625,12 +692,30 @@ static __global__ __launch_bounds__(256) void member_place_kernel(
   for (int e = tid; e < nblk; e += blockDim.x) sthr[e] = thr[e];
   __syncthreads();
   const int tile = blockIdx.x;
-  const long long c0 = (long long)tile * T2;
-  const long long c1 = min((long long)n, c0 + T2);
-  const long long q = (c1 - c0 + 3) / 4;
+  const long long c0 = (long long)tile * kMemberTile;
+  const long long c1 = min((long long)n, c0 + kMemberTile);
+  const long long q = (c1 - c0 + 3) / 4;  // <= kSteps * 64
   const long long w0 = c0 + w * q, w1 = min(c1, w0 + q);
-  for (long long i = w0 + lane; i < w1; i += 64)
-    atomicAdd(&slot[w * nseg + block_of(sthr, nblk, kscale, keys[i]) * nb + cats[i]], 1);
+  // segment of each of the wave's cells (-1 past its quarter): all loads
+  // first, kept in registers for both passes
+  constexpr int kSteps = kMemberTile / 256;
+  int sgv[kSteps];
+  {
+    uint32_t kv[kSteps];
+    int cv[kSteps];
+#pragma unroll
+    for (int u = 0; u < kSteps; ++u) {
+      const long long i = w0 + 64 * u + lane;
+      kv[u] = i < w1 ? keys[i] : 0u;
+      cv[u] = i < w1 ? cats[i] : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < kSteps; ++u)
+      sgv[u] = cv[u] >= 0 ? block_of(sthr, nblk, kscale, kv[u]) * nb + cv[u] : -1;
+  }
+#pragma unroll
+  for (int u = 0; u < kSteps; ++u)
+    if (sgv[u] >= 0) atomicAdd(&slot[w * nseg + sgv[u]], 1);
   __syncthreads();
   for (int e = tid; e < nseg; e += blockDim.x) {
     int run = tbase[(long long)e * tiles + tile];
@@ -642,10 +727,11 @@ static __global__ __launch_bounds__(256) void member_place_kernel(
   }
   __syncthreads();
   int* myslot = slot + w * nseg;
-  for (long long base = w0; base < w1; base += 64) {
-    const long long i = base + lane;
-    const bool valid = i < w1;
-    const int sg = valid ? block_of(sthr, nblk, kscale, keys[i]) * nb + cats[i] : 0;
+#pragma unroll
+  for (int u = 0; u < kSteps; ++u) {
+    const long long i = w0 + 64 * u + lane;
+    const bool valid = sgv[u] >= 0;
+    const int sg = valid ? sgv[u] : 0;
     // lanes with this lane's segment: agree on every segment bit
     unsigned long long match = __ballot(valid);
     for (int b = 0; b < sbits; ++b) {
@@ -742,8 +828,13 @@ static int launch_member_order(const PermDims& p, uint32_t seed, const int* cats
     hipLaunchKernelGGL(bound_mark_kernel, dim3(1, I), dim3(256), 0, s, b.bstart, bmark, b.bcnt,
                        nbk, p.nblk, p.bs, st);
     HZ_LAUNCHED("bound_mark");
-    hipLaunchKernelGGL(bound_collect_kernel, dim3(g, I), dim3(256), 0, s, b.keys, bmark, b.bcnt,
-                       b.blist, p.n, shift, st);
+    // marked buckets hold a fraction f = min(nblk - 1, nbk) / nbk of the keys:
+    // slices of about kCollectCap / (4 f) cells
+    const double f = (double)std::min(p.nblk - 1, nbk) / (double)nbk;
+    const int slice = (int)std::max(256.0, std::min(16384.0, kCollectCap / (4.0 * f)));
+    hipLaunchKernelGGL(bound_collect_kernel, dim3((int)ceil_div(p.n, slice), I), dim3(256),
+                       (size_t)2 * p.nblk * sizeof(int), s, b.keys, bmark, b.bcnt, b.blist, p.n,
+                       shift, nbk, p.nblk, slice, err, st);
     HZ_LAUNCHED("bound_collect");
     hipLaunchKernelGGL(bound_select_kernel, dim3(p.nblk - 1, I), dim3(256), 0, s, b.bstart,
                        bmark, b.bcnt, b.blist, b.thr, nbk, p.bs, err, st);
@@ -755,7 +846,7 @@ static int launch_member_order(const PermDims& p, uint32_t seed, const int* cats
   const float kscale = (float)((double)p.n / ((double)p.bs * 4294967296.0));
   hipLaunchKernelGGL(member_count_kernel, dim3(tiles, I), dim3(256),
                      (size_t)(nseg + p.nblk) * sizeof(int), s, b.keys, b.thr, cats, p.n, p.nblk,
-                     p.nb, tiles, T2, kscale, b.thist, st);
+                     p.nb, tiles, kscale, b.thist, st);
   HZ_LAUNCHED("member_count");
   hipLaunchKernelGGL(chunk_table_kernel, dim3(1, I), dim3(kScanThreads), 0, s, nullptr, b.thist,
                      b.tbase, nseg, tiles, p.n, p.chunk, t.seg_start, t.seg_ch, t.ch_start,
@@ -767,7 +858,7 @@ static int launch_member_order(const PermDims& p, uint32_t seed, const int* cats
   HZ_LAUNCHED("chunk_fill");
   hipLaunchKernelGGL(member_place_kernel, dim3(tiles, I), dim3(256),
                      (size_t)(4 * nseg + p.nblk) * sizeof(int), s, b.keys, b.thr, cats, p.n,
-                     p.nblk, p.nb, tiles, T2, kscale, b.tbase, t.order, st);
+                     p.nblk, p.nb, tiles, kscale, b.tbase, t.order, st);
   HZ_LAUNCHED("member_place");
   return HZ_OK;
 }
