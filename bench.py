@@ -646,7 +646,7 @@ def main() -> int:
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--fp64-steps", type=int, default=2,
                     help="timed steps of the FP64 leg of the default run (0 = skip)")
-    ap.add_argument("--batch-copies", type=int, default=2,
+    ap.add_argument("--batch-copies", type=int, default=4,
                     help="instances of the workload in the batched leg of the default run "
                          "(1 = skip)")
     ap.add_argument("--served-leg", type=float, default=12.0,

@@ -951,6 +951,26 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
   ia.nseg = (int)nseg;
   ia.K = L.K;
   ia.scale_o = L.scale_o;
+  ia.lv = levels_of(L);
+  // Several covariates run one launch per block; their centroid pass uses
+  // the streaming kernel and the old rows' level sums come from insum_kernel
+  // (segments are small groups of cells, which the ordered centroid pass
+  // handles poorly).
+  bool blk_stream = false;
+  if constexpr (std::is_same<T, float>::value) {
+    blk_stream = use_cm && L.C > 1 && L.mfma && L.K <= 256 &&
+                 reinterpret_cast<uintptr_t>(Z) % 16 == 0 &&
+                 reinterpret_cast<uintptr_t>(R) % 16 == 0;
+    if (blk_stream) {
+      cs.Z = Z;
+      cs.R = R;
+      cs.Ypart = reinterpret_cast<float*>(L.Ypart);
+      cs.ctrl = L.ctrl;
+      cs.n = L.n;
+      cs.d = L.d;
+      cs.K = L.K;
+    }
+  }
   // Block orders of L.nord iterations at a time, in one set of launches on
   // the caller's stream ahead of those iterations: built beside the sweep
   // (on the side stream) they would slow it, since every block waits for its
@@ -1036,16 +1056,24 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
     if (!iter_done) {
       HZ_HIP(hipStreamWaitEvent(s, ss->ready[it & 1], 0));
       HZ_TRY(fill<long long>(L.acc_in, L.nacc * L.K, 0, s));
+      if (blk_stream) {
+        ia.order = t.order;
+        ia.seg_start = t.seg_start;
+        ProfScope ps(HZ_PROF_ORDER, s);
+        HZ_TRY(launch_insum(L, ia, s));
+      }
       {
         ProfScope ps(HZ_PROF_CENTROID, s);
-        if (use_cm)
+        if (blk_stream)
+          HZ_TRY(launch_cent_stream(L, cs, L.Gs, s));
+        else if (use_cm)
           HZ_TRY(launch_cent_mfma(L, cm, L.Gm, s));
         else
           HZ_TRY(launch_centroid<T>(L, ca, L.G, s));
       }
       {
         ProfScope ps(HZ_PROF_YREDUCE, s);
-        ya.G = use_cm ? L.Gm : L.G;
+        ya.G = blk_stream ? L.Gs : (use_cm ? L.Gm : L.G);
         ya.setup_block0 = 1;
         hipLaunchKernelGGL(yreduce_kernel<T>, ygrid, dim3(kBlock), 0, s, ya);
         HZ_LAUNCHED("yreduce_kernel");

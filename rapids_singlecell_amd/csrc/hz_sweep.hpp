@@ -49,16 +49,19 @@ struct InSumArgs {
   const float* R;
   const int* order;
   const int* seg_start;  // nseg + 1 positions (into order) where each segment starts
-  long long* acc_in;     // nseg x K, zero at launch
+  long long* acc_in;     // accumulator rows x K, zero at launch
   const int* done;       // convergence flag: nothing to do once set
   int n, nseg, K;
   double scale_o;
+  Levels lv;             // accumulator rows of each segment (one per covariate)
 };
 
 constexpr int kInSumSpan = 256;  // positions of the order per wave
 
 // Sums of the rows of R (as left by the previous clustering iteration) over
-// every (block, batch) segment of this iteration's block order: the "remove
+// every (block, batch) segment of this iteration's block order (with several
+// covariates, every (block, group) segment, added to the accumulator row of
+// each of the group's levels): the "remove
 // the block's old contribution" terms of clustering.cu:321-334, for all blocks
 // at once.  Each wave walks a fixed span of consecutive positions of the
 // grouped order (segment-major), 16 rows at a time with every row's loads
@@ -111,7 +114,9 @@ __global__ __launch_bounds__(kBlock) void insum_kernel(InSumArgs a) {
 #pragma unroll
     for (int w = 0; w < KW; ++w) {
       const int k = lane + kWave * w;
-      if (k < K) fixed_add(&a.acc_in[(long long)seg * K + k], (double)ks[w], a.scale_o);
+      if (k < K)
+        for (int cv = 0; cv < a.lv.count(); ++cv)
+          fixed_add(&a.acc_in[a.lv.row(seg, cv) * K + k], (double)ks[w], a.scale_o);
     }
     p = pe;
     ++seg;
