@@ -128,26 +128,40 @@ __device__ T block_update(const EpiArgs<T>& a, unsigned char* smem, int smem_byt
     const int ks = min(KS, K - k0);
     const int n = B * ks;
     __syncthreads();
-    for (int q = tid; q < n; q += nt) {
-      const int b = q / ks, kk = q - b * ks;
-      const long long p = (long long)b * K + k0 + kk;
-      double vo = 0.0, vi = 0.0;
-      if (flags & (kUpdOut | kUpdOverwrite)) {
-        const long long* src = a.acc_out + (long long)(seg_out + b) * K + k0 + kk;
-        long long q = 0;
+    // the replica loads of kUpdU entries per thread are issued together: the
+    // device-scope loads go past L2, so a loop of one entry at a time would
+    // wait one fabric round trip per entry (the entry index is clamped so
+    // that every load is valid; only entries below n are stored)
+    constexpr int kUpdU = 4;
+    for (int q0 = tid; q0 < n; q0 += kUpdU * nt) {
+      long long qo[kUpdU];
+      long long qi[kUpdU];
 #pragma unroll
-        for (int r = 0; r < kRep; ++r) q += load_agent(src + r * a.rep_stride);
-        vo = (double)q * inv;
+      for (int u = 0; u < kUpdU; ++u) {
+        const int q = min(q0 + u * nt, n - 1);
+        const int b = q / ks, kk = q - b * ks;
+        qo[u] = 0;
+        qi[u] = 0;
+        if (flags & (kUpdOut | kUpdOverwrite)) {
+          const long long* src = a.acc_out + (long long)(seg_out + b) * K + k0 + kk;
+#pragma unroll
+          for (int r = 0; r < kRep; ++r) qo[u] += load_agent(src + r * a.rep_stride);
+        }
+        if (flags & kUpdIn) qi[u] = a.acc_in[(long long)(seg_in + b) * K + k0 + kk];  // an earlier launch
       }
-      if (flags & kUpdIn) {
-        const long long* src = a.acc_in + (long long)(seg_in + b) * K + k0 + kk;
-        vi = (double)(*src) * inv;  // written by an earlier launch
-      }
-      s_do[q] = vo;
-      s_di[q] = vi;
-      if (!(flags & kUpdOverwrite)) {
-        s_O[q] = a.O[p];
-        s_E[q] = a.E[p];
+#pragma unroll
+      for (int u = 0; u < kUpdU; ++u) {
+        const int q = q0 + u * nt;
+        if (q < n) {
+          const int b = q / ks, kk = q - b * ks;
+          const long long p = (long long)b * K + k0 + kk;
+          s_do[q] = (double)qo[u] * inv;
+          s_di[q] = (double)qi[u] * inv;
+          if (!(flags & kUpdOverwrite)) {
+            s_O[q] = a.O[p];
+            s_E[q] = a.E[p];
+          }
+        }
       }
     }
     __syncthreads();
