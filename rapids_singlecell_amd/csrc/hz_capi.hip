@@ -1009,10 +1009,9 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
       // beside the centroid pass (only that kernel waits on the side stream)
       {
         ProfScope ps(HZ_PROF_ORDER, s);
-        HZ_TRY(fill<long long>(L.acc_in, L.nacc * L.K, 0, s));
         hipLaunchKernelGGL(sweep_count_kernel, dim3(L.pd.nblk), dim3(kBlock),
                            (size_t)L.B * kSweepRep * sizeof(int), s, &L.ctrl->done, t.seg_ch,
-                           t.ch_seg, L.swcnt, L.B, sweep_grid);
+                           t.ch_seg, L.swcnt, L.B, sweep_grid, L.acc_in, L.K);
         HZ_LAUNCHED("sweep_count_kernel");
       }
       HZ_HIP(hipEventRecord(ss->freed[it & 1], s));
@@ -1327,10 +1326,9 @@ static int batched_impl(hz_job* jobs, int n, void* bws, size_t bws_bytes, hipStr
       Layout& L = b.L;
       const ChunkTab t = tab_of(L.ct, L.pd, it % L.nord);
       ProfScope ps(HZ_PROF_ORDER, s);
-      HZ_TRY(fill<long long>(L.acc_in, L.nacc * L.K, 0, s));
       hipLaunchKernelGGL(sweep_count_kernel, dim3(L.pd.nblk), dim3(kBlock),
                          (size_t)L.B * kSweepRep * sizeof(int), s, &L.ctrl->done, t.seg_ch,
-                         t.ch_seg, L.swcnt, L.B, b.G);
+                         t.ch_seg, L.swcnt, L.B, b.G, L.acc_in, L.K);
       HZ_LAUNCHED("sweep_count_kernel");
     }
     HZ_HIP(hipEventRecord(ss->freed[it & 1], s));

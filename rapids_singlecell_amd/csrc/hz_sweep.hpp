@@ -129,14 +129,20 @@ __global__ __launch_bounds__(kBlock) void insum_kernel(InSumArgs a) {
 // of block j, batch b, replica r: workgroup w owns the contiguous chunk range
 // [nlo + w*per, min(nhi, nlo + (w+1)*per)) of block j and adds to every batch
 // from its first chunk's to its last chunk's, in replica w % kSweepRep (the
-// same rule as the sweep's flush).  One workgroup per block.
+// same rule as the sweep's flush).  One workgroup per block, which also
+// clears block j's rows of the old-row sums (acc_in, B x K per block) when
+// acc_in is given.
 static __global__ __launch_bounds__(kBlock) void sweep_count_kernel(const int* __restrict__ done,
                                                              const int* __restrict__ seg_ch,
                                                              const int* __restrict__ ch_seg,
-                                                             int* __restrict__ cnt, int B, int G) {
+                                                             int* __restrict__ cnt, int B, int G,
+                                                             long long* __restrict__ acc_in,
+                                                             int K) {
   extern __shared__ int s_c[];  // B x kSweepRep
   if (*done) return;
   const int j = blockIdx.x;
+  if (acc_in)
+    for (int i = threadIdx.x; i < B * K; i += blockDim.x) acc_in[(long long)j * B * K + i] = 0;
   for (int i = threadIdx.x; i < B * kSweepRep; i += blockDim.x) s_c[i] = 0;
   __syncthreads();
   const int nlo = seg_ch[j * B], nhi = seg_ch[j * B + B];
