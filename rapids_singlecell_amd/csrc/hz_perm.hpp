@@ -672,7 +672,13 @@ static __global__ __launch_bounds__(256) void member_count_kernel(
 // the waves before it.  Per 64 cells, the lanes holding the same segment are
 // found with one ballot per segment bit, so a lane's rank among them (lanes
 // below it) and their count take a few bit counts whatever the number of
-// distinct segments.  LDS: 4 x nseg slots, then nblk thresholds.
+// distinct segments.  With kStage, the tile's cells are first placed in LDS
+// in (segment, cell) order, then written out so that the cells of one
+// segment go to consecutive addresses (runs of about 8192 / nseg cells)
+// instead of one scattered 4-byte store each.
+// LDS: 4 x nseg slots, [kStage: nseg offsets], nblk thresholds,
+// [kStage: 2 x kMemberTile 16-bit segment and cell entries].
+template <bool kStage>
 static __global__ __launch_bounds__(256) void member_place_kernel(
     const uint32_t* __restrict__ keys, const uint32_t* __restrict__ thr,
     const int* __restrict__ cats, int n, int nblk, int nb, int tiles, float kscale,
@@ -685,8 +691,11 @@ static __global__ __launch_bounds__(256) void member_place_kernel(
   order += st.n * y;
   const int nseg = nblk * nb;
   const int sbits = 32 - __builtin_clz((unsigned)max(nseg - 1, 1));  // bits of a segment id
-  int* slot = lds;  // [wave][segment]
-  uint32_t* sthr = reinterpret_cast<uint32_t*>(lds + 4 * nseg);
+  int* slot = lds;                   // [wave][segment]
+  int* delta = slot + 4 * nseg;      // kStage: global minus local position of each segment
+  uint32_t* sthr = reinterpret_cast<uint32_t*>(lds + (kStage ? 5 : 4) * nseg);
+  unsigned short* lseg = reinterpret_cast<unsigned short*>(sthr + nblk);  // kStage
+  unsigned short* lcell = lseg + kMemberTile;                               // kStage
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   for (int e = tid; e < 4 * nseg; e += blockDim.x) slot[e] = 0;
   for (int e = tid; e < nblk; e += blockDim.x) sthr[e] = thr[e];
@@ -717,12 +726,43 @@ static __global__ __launch_bounds__(256) void member_place_kernel(
   for (int u = 0; u < kSteps; ++u)
     if (sgv[u] >= 0) atomicAdd(&slot[w * nseg + sgv[u]], 1);
   __syncthreads();
-  for (int e = tid; e < nseg; e += blockDim.x) {
-    int run = tbase[(long long)e * tiles + tile];
-    for (int u = 0; u < 4; ++u) {
-      const int c = slot[u * nseg + e];
-      slot[u * nseg + e] = run;
-      run += c;
+  if constexpr (kStage) {
+    // local start of each segment: exclusive scan of the tile's counts
+    // (wave 0, 64 segments at a time)
+    if (w == 0) {
+      int carry = 0;
+      for (int e0 = 0; e0 < nseg; e0 += 64) {
+        const int e = e0 + lane;
+        const int tot = (e < nseg) ? slot[e] + slot[nseg + e] + slot[2 * nseg + e] + slot[3 * nseg + e]
+                                   : 0;
+        int x = tot;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+          const int v = __shfl_up(x, off, 64);
+          if (lane >= off) x += v;
+        }
+        if (e < nseg) delta[e] = carry + x - tot;
+        carry += __shfl(x, 63, 64);
+      }
+    }
+    __syncthreads();
+    for (int e = tid; e < nseg; e += blockDim.x) {
+      int run = delta[e];
+      delta[e] = tbase[(long long)e * tiles + tile] - run;
+      for (int u = 0; u < 4; ++u) {
+        const int c = slot[u * nseg + e];
+        slot[u * nseg + e] = run;
+        run += c;
+      }
+    }
+  } else {
+    for (int e = tid; e < nseg; e += blockDim.x) {
+      int run = tbase[(long long)e * tiles + tile];
+      for (int u = 0; u < 4; ++u) {
+        const int c = slot[u * nseg + e];
+        slot[u * nseg + e] = run;
+        run += c;
+      }
     }
   }
   __syncthreads();
@@ -745,9 +785,19 @@ static __global__ __launch_bounds__(256) void member_place_kernel(
     // it past the group (LDS operations of a wave complete in order)
     const int start = valid ? myslot[sg] : 0;
     if (valid) {
-      order[start + rank] = (int)i;
+      if constexpr (kStage) {
+        lseg[start + rank] = (unsigned short)sg;
+        lcell[start + rank] = (unsigned short)(i - c0);
+      } else {
+        order[start + rank] = (int)i;
+      }
       if (rank == 0) myslot[sg] = start + __popcll(match);
     }
+  }
+  if constexpr (kStage) {
+    __syncthreads();
+    const int ncell = (int)(c1 - c0);
+    for (int p = tid; p < ncell; p += blockDim.x) order[p + delta[lseg[p]]] = (int)(c0 + lcell[p]);
   }
 }
 
@@ -856,9 +906,17 @@ static int launch_member_order(const PermDims& p, uint32_t seed, const int* cats
                      dim3(256), 0, s, nullptr, nseg, p.chunk, t.seg_start, t.seg_ch, t.ch_start,
                      t.ch_len, t.ch_seg, p.maxch, st);
   HZ_LAUNCHED("chunk_fill");
-  hipLaunchKernelGGL(member_place_kernel, dim3(tiles, I), dim3(256),
-                     (size_t)(4 * nseg + p.nblk) * sizeof(int), s, b.keys, b.thr, cats, p.n,
-                     p.nblk, p.nb, tiles, kscale, b.tbase, t.order, st);
+  // the staged placement when its LDS fits 64 KiB (16-bit segment ids)
+  const size_t lds_staged = (size_t)(5 * nseg + p.nblk) * sizeof(int) +
+                            (size_t)2 * kMemberTile * sizeof(unsigned short);
+  if (lds_staged <= 65536 && nseg <= 65536)
+    hipLaunchKernelGGL(member_place_kernel<true>, dim3(tiles, I), dim3(256), lds_staged, s,
+                       b.keys, b.thr, cats, p.n, p.nblk, p.nb, tiles, kscale, b.tbase, t.order,
+                       st);
+  else
+    hipLaunchKernelGGL(member_place_kernel<false>, dim3(tiles, I), dim3(256),
+                       (size_t)(4 * nseg + p.nblk) * sizeof(int), s, b.keys, b.thr, cats, p.n,
+                       p.nblk, p.nb, tiles, kscale, b.tbase, t.order, st);
   HZ_LAUNCHED("member_place");
   return HZ_OK;
 }
