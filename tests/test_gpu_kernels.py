@@ -234,6 +234,17 @@ def test_block_membership_bit_exact(device, n, B, seed):
     _check_membership(n, bs, cats, seed, blk, order)
 
 
+def test_block_membership_many_segments(device):
+    """20 blocks x 100 batches = 2000 segments: the membership kernels still
+    fit, but the placement's LDS staging does not, so the cells are placed
+    with one store each; same membership."""
+    n, B = 200_000, 100
+    cats = np.random.default_rng(5).integers(0, B, n).astype(np.int32)
+    bs = int(n * 0.05)
+    blk, order = _membership(device, n, bs, cats, 4242, 3)
+    _check_membership(n, bs, cats, 4242, blk, order)
+
+
 @pytest.mark.parametrize("n", [9_000_000, 17_000_000])
 def test_block_membership_large(device, n):
     """Past 2^23 cells the boundary buckets' histogram takes the whole 64 KiB
