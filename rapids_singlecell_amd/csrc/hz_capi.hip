@@ -432,17 +432,23 @@ static int launch_centroid(const Layout& L, const CentArgs<T>& a, int grid, hipS
   }
 }
 
-static int launch_insum(const Layout& L, const InSumArgs& a, hipStream_t s) {
+template <typename T>
+static int launch_insum_t(const Layout& L, const InSumArgs& a, hipStream_t s) {
   const int grid = (int)std::max<long long>(1, ceil_div(ceil_div(L.n, kInSumSpan), kBlock / kWave));
   switch ((int)ceil_div(L.K, kWave)) {
-    case 1: hipLaunchKernelGGL(insum_kernel<1>, dim3(grid), dim3(kBlock), 0, s, a); break;
-    case 2: hipLaunchKernelGGL(insum_kernel<2>, dim3(grid), dim3(kBlock), 0, s, a); break;
-    case 3: hipLaunchKernelGGL(insum_kernel<3>, dim3(grid), dim3(kBlock), 0, s, a); break;
-    case 4: hipLaunchKernelGGL(insum_kernel<4>, dim3(grid), dim3(kBlock), 0, s, a); break;
+    case 1: hipLaunchKernelGGL((insum_kernel<T, 1>), dim3(grid), dim3(kBlock), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((insum_kernel<T, 2>), dim3(grid), dim3(kBlock), 0, s, a); break;
+    case 3: hipLaunchKernelGGL((insum_kernel<T, 3>), dim3(grid), dim3(kBlock), 0, s, a); break;
+    case 4: hipLaunchKernelGGL((insum_kernel<T, 4>), dim3(grid), dim3(kBlock), 0, s, a); break;
     default: set_error("old-row sums support at most 256 clusters"); return HZ_ERR_UNSUPPORTED;
   }
   HZ_LAUNCHED("insum_kernel");
   return HZ_OK;
+}
+
+// old-row sums of R rows of the job's element type
+static int launch_insum(const Layout& L, const InSumArgs& a, hipStream_t s) {
+  return L.esz == 8 ? launch_insum_t<double>(L, a, s) : launch_insum_t<float>(L, a, s);
 }
 
 static int launch_split(const Layout& L, const Ctrl* ctrl, hipStream_t s) {
@@ -944,7 +950,7 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
   HZ_TRY(side_stream(&ss));
   const hipStream_t s2 = ss->s;
   InSumArgs ia{};
-  ia.R = reinterpret_cast<const float*>(R);
+  ia.R = R;
   ia.done = &L.ctrl->done;
   ia.acc_in = L.acc_in;
   ia.n = L.n;
@@ -956,6 +962,10 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
   // the streaming kernel and the old rows' level sums come from insum_kernel
   // (segments are small groups of cells, which the ordered centroid pass
   // handles poorly).
+  // FP64 (and any job without the MFMA centroid pass): the centroids come
+  // from a natural-order pass (contiguous rows) and the old rows' sums from
+  // insum_kernel, instead of one pass over the grouped order doing both
+  const bool blk_natural = !use_cm && L.K <= 256;
   bool blk_stream = false;
   if constexpr (std::is_same<T, float>::value) {
     blk_stream = use_cm && L.C > 1 && L.mfma && L.K <= 256 &&
@@ -1055,7 +1065,7 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
     if (!iter_done) {
       HZ_HIP(hipStreamWaitEvent(s, ss->ready[it & 1], 0));
       HZ_TRY(fill<long long>(L.acc_in, L.nacc * L.K, 0, s));
-      if (blk_stream) {
+      if (blk_stream || blk_natural) {
         ia.order = t.order;
         ia.seg_start = t.seg_start;
         ProfScope ps(HZ_PROF_ORDER, s);
@@ -1063,12 +1073,18 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
       }
       {
         ProfScope ps(HZ_PROF_CENTROID, s);
-        if (blk_stream)
+        if (blk_stream) {
           HZ_TRY(launch_cent_stream(L, cs, L.Gs, s));
-        else if (use_cm)
+        } else if (use_cm) {
           HZ_TRY(launch_cent_mfma(L, cm, L.Gm, s));
-        else
+        } else if (blk_natural) {
+          CentArgs<T> can = ca;
+          can.order = nullptr;
+          can.acc_in = nullptr;
+          HZ_TRY(launch_centroid<T>(L, can, L.G, s));
+        } else {
           HZ_TRY(launch_centroid<T>(L, ca, L.G, s));
+        }
       }
       {
         ProfScope ps(HZ_PROF_YREDUCE, s);

@@ -324,6 +324,9 @@ struct CentArgs {
 // LDS, the next tile prefetched into registers), plus each (block, batch)
 // segment's column sums of R into acc_in.  The column sums come out of the
 // same accumulation: column d of the Z tile holds 1 for every valid row.
+// Without an order (order == nullptr) the chunks are the cells' natural
+// order in 64-row pieces, one segment, and acc_in (nullptr) is not written:
+// Y alone, with contiguous loads.
 template <typename T, int KT, int DT>
 __global__ __launch_bounds__(kBlock) void centroid_kernel(CentArgs<T> a) {
   constexpr int Kp = 16 * KT, Dp = 16 * DT;
@@ -337,9 +340,13 @@ __global__ __launch_bounds__(kBlock) void centroid_kernel(CentArgs<T> a) {
   if (a.ctrl->done) return;
   const int tid = threadIdx.x, kg = tid >> 4, dg = tid & 15;
   const int d = a.d, K = a.K;
-  const int nch = a.seg_ch[a.nseg];
+  const bool natural = a.order == nullptr;
+  const int nch = natural ? (a.n + 63) / 64 : a.seg_ch[a.nseg];
   const int c0 = blockIdx.x * a.cpw;
   const int c1 = min(nch, c0 + a.cpw);
+  auto ch_start = [&](int c) { return natural ? 64 * c : a.ch_start[c]; };
+  auto ch_len = [&](int c) { return natural ? min(64, a.n - 64 * c) : a.ch_len[c]; };
+  auto cell_at = [&](int p) -> long long { return natural ? (long long)p : (long long)a.order[p]; };
 
   T acc[KT][DT];
 #pragma unroll
@@ -350,14 +357,14 @@ __global__ __launch_bounds__(kBlock) void centroid_kernel(CentArgs<T> a) {
   T rv[MR], zv[MZ];
   // tile cursor: chunk c, row offset r0 inside the chunk
   auto fetch = [&](int c, int r0) {
-    const int p0 = a.ch_start[c] + r0;
-    const int rows = min(TR, a.ch_len[c] - r0);
+    const int p0 = ch_start(c) + r0;
+    const int rows = min(TR, ch_len(c) - r0);
 #pragma unroll
     for (int m = 0; m < MR; ++m) {
       const int e = tid + m * kBlock;
       const int r = e / Kp, k = e % Kp;
       T v = T(0);
-      if (r < rows && k < K) v = a.R[(long long)a.order[p0 + r] * K + k];
+      if (r < rows && k < K) v = a.R[cell_at(p0 + r) * K + k];
       rv[m] = v;
     }
 #pragma unroll
@@ -367,7 +374,7 @@ __global__ __launch_bounds__(kBlock) void centroid_kernel(CentArgs<T> a) {
       T v = T(0);
       if (r < rows) {
         if (cc < d)
-          v = a.Z[(long long)a.order[p0 + r] * d + cc];
+          v = a.Z[cell_at(p0 + r) * d + cc];
         else if (cc == d)
           v = T(1);
       }
@@ -378,8 +385,8 @@ __global__ __launch_bounds__(kBlock) void centroid_kernel(CentArgs<T> a) {
   int c = c0, r0 = 0;
   if (c < c1) fetch(c, 0);
   while (c < c1) {
-    const int seg = a.ch_seg[c];
-    const int rows = min(TR, a.ch_len[c] - r0);
+    const int seg = natural ? 0 : a.ch_seg[c];
+    const int rows = min(TR, ch_len(c) - r0);
     __syncthreads();
 #pragma unroll
     for (int m = 0; m < MR; ++m) Rs[tid + m * kBlock] = rv[m];
@@ -388,7 +395,7 @@ __global__ __launch_bounds__(kBlock) void centroid_kernel(CentArgs<T> a) {
     __syncthreads();
     // advance the cursor and prefetch the next tile
     int nc = c, nr0 = r0 + TR;
-    if (nr0 >= a.ch_len[c]) {
+    if (nr0 >= ch_len(c)) {
       nc = c + 1;
       nr0 = 0;
     }
@@ -405,7 +412,8 @@ __global__ __launch_bounds__(kBlock) void centroid_kernel(CentArgs<T> a) {
         for (int j = 0; j < DT; ++j) acc[i][j] += rr[i] * zz[j];
     }
     // segment finished: flush the ones-column (column sums of R) to acc_in
-    const bool seg_end = (nc >= c1) || (nc != c && a.ch_seg[nc] != seg);
+    const bool seg_end =
+        a.acc_in && ((nc >= c1) || (nc != c && (natural ? 0 : a.ch_seg[nc]) != seg));
     if (seg_end) {
 #pragma unroll
       for (int j = 0; j < DT; ++j) {

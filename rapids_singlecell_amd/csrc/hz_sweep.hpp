@@ -46,7 +46,7 @@ constexpr int kSweepRep = 1;
 // ---------------------------------------------------------------- old-row sums
 
 struct InSumArgs {
-  const float* R;
+  const void* R;         // float or double rows (the kernel's element type)
   const int* order;
   const int* seg_start;  // nseg + 1 positions (into order) where each segment starts
   long long* acc_in;     // accumulator rows x K, zero at launch
@@ -67,11 +67,12 @@ constexpr int kInSumSpan = 256;  // positions of the order per wave
 // grouped order (segment-major), 16 rows at a time with every row's loads
 // issued together, and adds its per-segment column sums to the fixed-point
 // accumulators once per segment it touches (lanes own columns).
-template <int KW>
+template <typename T, int KW>
 __global__ __launch_bounds__(kBlock) void insum_kernel(InSumArgs a) {
   const int lane = threadIdx.x & (kWave - 1);
   if (*a.done) return;
   const int K = a.K;
+  const T* __restrict__ R = static_cast<const T*>(a.R);
   const long long wv = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int p0 = (int)(wv * kInSumSpan);
   if (p0 >= a.n) return;
@@ -89,26 +90,26 @@ __global__ __launch_bounds__(kBlock) void insum_kernel(InSumArgs a) {
   int p = p0;
   while (p < p1) {
     const int pe = min(p1, a.seg_start[seg + 1]);
-    float ks[KW];
+    T ks[KW];
 #pragma unroll
-    for (int w = 0; w < KW; ++w) ks[w] = 0.f;
+    for (int w = 0; w < KW; ++w) ks[w] = T(0);
     for (int r0 = p; r0 < pe; r0 += 16) {
       const int mine = r0 + (lane & 15);
       const int cell = a.order[min(mine, pe - 1)];  // always a valid position
-      float x[16][KW];
+      T x[16][KW];
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int rc = __shfl(cell, r, kWave);
-        const unsigned base = (unsigned)rc * (unsigned)K;
+        const long long base = (long long)rc * K;
 #pragma unroll
-        for (int w = 0; w < KW; ++w) x[r][w] = a.R[base + (unsigned)min(lane + kWave * w, K - 1)];
+        for (int w = 0; w < KW; ++w) x[r][w] = R[base + min(lane + kWave * w, K - 1)];
       }
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         // multiplicative mask: the (always valid) loads stay unconditional
-        const float m = (r0 + r < pe) ? 1.f : 0.f;
+        const T m = (r0 + r < pe) ? T(1) : T(0);
 #pragma unroll
-        for (int w = 0; w < KW; ++w) ks[w] = __builtin_fmaf(x[r][w], m, ks[w]);
+        for (int w = 0; w < KW; ++w) ks[w] = fma(x[r][w], m, ks[w]);
       }
     }
 #pragma unroll
