@@ -381,8 +381,10 @@ static int check_ws(const Layout& L, void* ws, size_t ws_bytes) {
 
 template <typename T>
 static size_t row_lds_bytes(const Layout& L) {
-  size_t need = (size_t)L.d * L.Kp * sizeof(T) + (size_t)L.d * kZtStride * sizeof(T) +
-                (size_t)(kBlock / kWave) * L.Kp * sizeof(double);
+  // centroids, then the Z tile, which the per-wave column sums reuse
+  size_t need = (size_t)L.d * L.Kp * sizeof(T) +
+                std::max((size_t)L.d * kZtStride * sizeof(T),
+                         (size_t)(kBlock / kWave) * L.Kp * sizeof(double));
   return std::max(need, (size_t)block_update_min_lds<T>(L.BL));
 }
 

@@ -608,7 +608,9 @@ __global__ __launch_bounds__(kBlock) void row_kernel(RowArgs<T> a) {
   const int d = a.d, K = a.K, B = a.epi.lv.Bg;
   T* YTs = reinterpret_cast<T*>(smem);
   T* ZTs = YTs + (size_t)d * Kp;
-  double* dred = reinterpret_cast<double*>(ZTs + (size_t)d * kZtStride);
+  // the per-wave column sums alias the Z tile (free after the last tile):
+  // the LDS then holds two workgroups per CU for FP64 at d = 50, K = 100
+  double* dred = reinterpret_cast<double*>(ZTs);
 
   const int nch_lo = a.seg_ch[a.seg_base];
   const int nch_hi = a.seg_ch[a.seg_base + B];
