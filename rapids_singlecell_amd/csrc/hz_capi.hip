@@ -73,6 +73,21 @@ int prepare_lds(const void* kern, size_t lds) {
   return HZ_OK;
 }
 
+// Dynamic LDS of a row launch whose last workgroup runs the block update:
+// `base` (what the row work needs) widened towards `want` (the update's
+// columns in one slab, so fewer serial rounds of fabric loads) as far as the
+// workgroups of a `grid`-workgroup launch that share one CU still fit in its
+// LDS together.
+size_t widen_lds(const void* kern, size_t base, size_t want, long long grid) {
+  if (want <= base) return base;
+  hipFuncAttributes fa{};
+  if (hipFuncGetAttributes(&fa, kern) != hipSuccess) return base;
+  const long long per_cu = std::max(1LL, (grid + device_cu_count() - 1) / device_cu_count());
+  const size_t share = (size_t)device_lds_limit() / (size_t)per_cu;
+  const size_t cap = share > fa.sharedSizeBytes ? (share - fa.sharedSizeBytes) & ~(size_t)255 : 0;
+  return std::max(base, std::min(want, cap));
+}
+
 // ---------------------------------------------------------------- launch timing
 //
 // Opt-in HIP-event timing of the library's own launches (benchmarks only).
@@ -205,9 +220,14 @@ static int make_layout(const hz_dims* dm, void* base, Layout& L) {
   HZ_RET_IF(!L.CT, HZ_ERR_UNSUPPORTED, "n_clusters=%d > 256 is not supported", L.K);
   HZ_RET_IF(!L.DT, HZ_ERR_UNSUPPORTED, "n_pcs=%d > 127 is not supported", L.d);
   {
-    const int need = (L.esz == 4) ? block_update_min_lds<float>(L.BL) : block_update_min_lds<double>(L.BL);
-    HZ_RET_IF(need > kYRedLds, HZ_ERR_UNSUPPORTED, "%d batch levels are too many (limit %d)", L.BL,
-              (kYRedLds - 16) / (16 + 2 * L.esz));
+    auto need_of = [&](int b) {
+      return (L.esz == 4) ? block_update_min_lds<float>(b) : block_update_min_lds<double>(b);
+    };
+    if (need_of(L.BL) > kYRedLds) {
+      int lim = 0;
+      while (need_of(lim + 1) <= kYRedLds) ++lim;
+      HZ_RET_IF(true, HZ_ERR_UNSUPPORTED, "%d batch levels are too many (limit %d)", L.BL, lim);
+    }
   }
   L.Kp = 16 * L.CT;
   L.Dp = 16 * L.DT;
@@ -296,7 +316,7 @@ static int make_layout(const hz_dims* dm, void* base, Layout& L) {
     L.nch_tmp_p = c.take<int>(L.B + 1);
   }
   L.acc_in = c.take<long long>(L.nacc * L.K);
-  L.acc_out = c.take<long long>((long long)kRep * L.nacc * L.K);
+  L.acc_out = c.take<long long>((long long)kAccRep * L.nacc * L.K);
   L.obj_acc = c.take<long long>(2 * kRep);
   L.ctrl = c.take<Ctrl>(1);
   L.ctrl_static = c.take<Ctrl>(1);
@@ -389,8 +409,10 @@ static size_t row_lds_bytes(const Layout& L) {
 }
 
 template <typename T, int CT>
-static int launch_row_ct(const RowArgs<T>& a, int grid, size_t lds, hipStream_t s) {
+static int launch_row_ct(const Layout& L, RowArgs<T> a, int grid, size_t lds, hipStream_t s) {
   auto kern = row_kernel<T, CT>;
+  lds = widen_lds(reinterpret_cast<const void*>(kern), lds, block_update_full_lds<T>(L.BL, L.K), grid);
+  a.smem_bytes = (int)lds;
   HZ_TRY(prepare_lds(reinterpret_cast<const void*>(kern), lds));
   hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), lds, s, a);
   HZ_LAUNCHED("row_kernel");
@@ -405,9 +427,9 @@ static int launch_row(const Layout& L, RowArgs<T> a, int grid, hipStream_t s) {
             "row kernel needs %zu bytes of LDS (limit %d): reduce n_pcs or n_clusters", lds,
             device_lds_limit());
   switch (L.CT) {
-    case 4: return launch_row_ct<T, 4>(a, grid, lds, s);
-    case 8: return launch_row_ct<T, 8>(a, grid, lds, s);
-    default: return launch_row_ct<T, 16>(a, grid, lds, s);
+    case 4: return launch_row_ct<T, 4>(L, a, grid, lds, s);
+    case 8: return launch_row_ct<T, 8>(L, a, grid, lds, s);
+    default: return launch_row_ct<T, 16>(L, a, grid, lds, s);
   }
 }
 
@@ -659,7 +681,7 @@ static int init_impl(const hz_dims* dm, const T* Z, const T* Y0, const int* cats
   HZ_TRY(launch_static_chunks(L.n, L.B, L.sd.chunk, cat_offsets, cell_indices, L.st, L.nch_tmp_s, s));
   hipLaunchKernelGGL(ctrl_reset_kernel, dim3(1), dim3(64), 0, s, L.ctrl);
   HZ_LAUNCHED("ctrl_reset");
-  HZ_TRY(fill<long long>(L.acc_out, (long long)kRep * L.nacc * L.K, 0, s));
+  HZ_TRY(fill<long long>(L.acc_out, (long long)kAccRep * L.nacc * L.K, 0, s));
   HZ_TRY(fill<long long>(L.obj_acc, 2 * kRep, 0, s));
   HZ_TRY(fill<int>(L.counters, L.pd.nblk + 6, 0, s));
   hipLaunchKernelGGL(centroids_to_yt_kernel<T>, dim3(1), dim3(kBlock), 0, s, Y0,

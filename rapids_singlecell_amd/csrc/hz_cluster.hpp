@@ -38,6 +38,12 @@ struct Ctrl {
 // id mod kRep) keeps same-address atomics from serialising.  Readers sum the
 // replicas.
 constexpr int kRep = 8;
+// Replicas of the per-block row sums (acc_out).  Each row workgroup adds its
+// few segment rows once, at its end, so the adds rarely meet; the block
+// update reads every replica of every entry through the fabric from one
+// workgroup, so each replica lengthens that serial tail.  The sums are fixed
+// point: the replica count changes no result bit.
+constexpr int kAccRep = 2;
 
 // Several covariates (stacked one-hot Phi): the chunk segments are
 // block-major groups of cells that share all levels; every segment adds its
@@ -73,7 +79,7 @@ struct EpiArgs {
   const T* theta;
   T* pen;              // B x K penalty for the next block
   long long* acc_in;   // nseg x K
-  long long* acc_out;  // kRep x nseg x K
+  long long* acc_out;  // kAccRep x nseg x K
   long long* obj_acc;  // kRep x 2: k-means error, entropy (fixed point)
   long long rep_stride;  // nblk x B x K
   Ctrl* ctrl;
@@ -114,9 +120,17 @@ __device__ T block_update(const EpiArgs<T>& a, unsigned char* smem, int smem_byt
                           int seg_in, int flags, double* red) {
   const int B = a.B, K = a.K;
   const int tid = threadIdx.x, nt = blockDim.x;
+  // Pr and theta per level, then the column slabs
+  T* s_pr = reinterpret_cast<T*>(smem);
+  T* s_th = s_pr + B;
+  const int head = (2 * B * (int)sizeof(T) + 15) & ~15;
+  for (int b = tid; b < B; b += nt) {
+    s_pr[b] = a.Pr[b];
+    s_th[b] = a.theta[b];
+  }
   const int per_col = B * (16 + 2 * (int)sizeof(T)) + 16;
-  const int KS = max(1, min(K, smem_bytes / per_col));
-  double* s_do = reinterpret_cast<double*>(smem);
+  const int KS = max(1, min(K, (smem_bytes - head) / per_col));
+  double* s_do = reinterpret_cast<double*>(smem + head);
   double* s_di = s_do + (size_t)B * KS;
   double* s_co = s_di + (size_t)B * KS;
   double* s_ci = s_co + KS;
@@ -128,38 +142,45 @@ __device__ T block_update(const EpiArgs<T>& a, unsigned char* smem, int smem_byt
     const int ks = min(KS, K - k0);
     const int n = B * ks;
     __syncthreads();
-    // the replica loads of kUpdU entries per thread are issued together: the
-    // device-scope loads go past L2, so a loop of one entry at a time would
-    // wait one fabric round trip per entry (the entry index is clamped so
+    // every load of kUpdU entries per thread (sum replicas, old sums, O, E)
+    // is issued before any is used: the device-scope loads go past L2, so
+    // each batch costs one fabric round trip (the entry index is clamped so
     // that every load is valid; only entries below n are stored)
-    constexpr int kUpdU = 4;
+    constexpr int kUpdU = 8;
     for (int q0 = tid; q0 < n; q0 += kUpdU * nt) {
       long long qo[kUpdU];
       long long qi[kUpdU];
+      T vo[kUpdU];
+      T ve[kUpdU];
 #pragma unroll
       for (int u = 0; u < kUpdU; ++u) {
         const int q = min(q0 + u * nt, n - 1);
         const int b = q / ks, kk = q - b * ks;
+        const long long p = (long long)b * K + k0 + kk;
         qo[u] = 0;
         qi[u] = 0;
+        vo[u] = T(0);
+        ve[u] = T(0);
         if (flags & (kUpdOut | kUpdOverwrite)) {
           const long long* src = a.acc_out + (long long)(seg_out + b) * K + k0 + kk;
 #pragma unroll
-          for (int r = 0; r < kRep; ++r) qo[u] += load_agent(src + r * a.rep_stride);
+          for (int r = 0; r < kAccRep; ++r) qo[u] += load_agent(src + r * a.rep_stride);
         }
         if (flags & kUpdIn) qi[u] = a.acc_in[(long long)(seg_in + b) * K + k0 + kk];  // an earlier launch
+        if (!(flags & kUpdOverwrite)) {
+          vo[u] = a.O[p];
+          ve[u] = a.E[p];
+        }
       }
 #pragma unroll
       for (int u = 0; u < kUpdU; ++u) {
         const int q = q0 + u * nt;
         if (q < n) {
-          const int b = q / ks, kk = q - b * ks;
-          const long long p = (long long)b * K + k0 + kk;
           s_do[q] = (double)qo[u] * inv;
           s_di[q] = (double)qi[u] * inv;
           if (!(flags & kUpdOverwrite)) {
-            s_O[q] = a.O[p];
-            s_E[q] = a.E[p];
+            s_O[q] = vo[u];
+            s_E[q] = ve[u];
           }
         }
       }
@@ -178,7 +199,7 @@ __device__ T block_update(const EpiArgs<T>& a, unsigned char* smem, int smem_byt
     for (int q = tid; q < n; q += nt) {
       const int b = q / ks, kk = q - b * ks;
       const long long p = (long long)b * K + k0 + kk;
-      const T pr = a.Pr[b];
+      const T pr = s_pr[b], th = s_th[b];
       T o, e;
       if (flags & kUpdOverwrite) {
         o = T(s_do[q]);
@@ -197,11 +218,11 @@ __device__ T block_update(const EpiArgs<T>& a, unsigned char* smem, int smem_byt
       }
       a.O[p] = o;
       a.E[p] = e;
-      if (flags & kUpdPen) a.pen[p] = penalty_value(e, o, a.theta[b], a.stabilized);
+      if (flags & kUpdPen) a.pen[p] = penalty_value(e, o, th, a.stabilized);
       if (flags & (kUpdObj | kUpdObjInit)) {
         // sigma * theta_b * O * log(numer / (E + 1)), kernels_clustering.cuh:69-103
         T numer = a.stabilized ? (o + e + T(1)) : (o + T(1));
-        div += (double)(a.theta[b] * o * dlog(numer / (e + T(1))));
+        div += (double)(th * o * dlog(numer / (e + T(1))));
       }
     }
   }
@@ -239,9 +260,18 @@ __device__ T block_update(const EpiArgs<T>& a, unsigned char* smem, int smem_byt
   return s_obj;
 }
 
+// LDS of block_update for B levels (one column per slab): Pr and theta,
+// then per column the two sums of each level, O and E, and two column sums.
 template <typename T>
 __host__ __device__ inline int block_update_min_lds(int B) {
-  return B * (16 + 2 * (int)sizeof(T)) + 16;
+  return ((2 * B * (int)sizeof(T) + 15) & ~15) + B * (16 + 2 * (int)sizeof(T)) + 16;
+}
+
+// LDS of block_update for all K columns in one slab.
+template <typename T>
+__host__ __device__ inline size_t block_update_full_lds(int B, int K) {
+  return (size_t)((2 * B * (int)sizeof(T) + 15) & ~15) +
+         (size_t)K * (B * (16 + 2 * (int)sizeof(T)) + 16);
 }
 
 // ---------------------------------------------------------------- arrival
@@ -476,7 +506,7 @@ __global__ __launch_bounds__(kBlock) void yreduce_kernel(YRedArgs<T> a) {
   const int wid = blockIdx.y * gridDim.x + blockIdx.x;
   // clear this iteration's block accumulators (read by earlier launches only)
   {
-    const long long n_out = (long long)kRep * a.epi.rep_stride;
+    const long long n_out = (long long)kAccRep * a.epi.rep_stride;
     for (long long e = (long long)wid * blockDim.x + threadIdx.x; e < n_out;
          e += (long long)nwg * blockDim.x)
       a.epi.acc_out[e] = 0;
@@ -732,7 +762,7 @@ __global__ __launch_bounds__(kBlock) void row_kernel(RowArgs<T> a) {
       for (int q = 0; q < CT; ++q) dred[w * Kp + lane * CT + q] = dO[q];
     }
     __syncthreads();
-    const int rep = blockIdx.x % kRep;
+    const int rep = blockIdx.x % kAccRep, orep = blockIdx.x % kRep;
     unsigned long long sink = 0;
     for (int k = tid; k < K; k += kBlock) {
       double t = 0.0;
@@ -744,8 +774,8 @@ __global__ __launch_bounds__(kBlock) void row_kernel(RowArgs<T> a) {
     km = block_sum(km, red);
     ent = block_sum(ent, red);
     if (tid == 0) {
-      sink += fixed_add_wait(&a.epi.obj_acc[2 * rep], km, a.scale_obj);
-      sink += fixed_add_wait(&a.epi.obj_acc[2 * rep + 1], ent, a.scale_obj);
+      sink += fixed_add_wait(&a.epi.obj_acc[2 * orep], km, a.scale_obj);
+      sink += fixed_add_wait(&a.epi.obj_acc[2 * orep + 1], ent, a.scale_obj);
     }
     asm volatile("" ::"v"(sink));
   }

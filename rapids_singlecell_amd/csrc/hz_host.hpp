@@ -71,6 +71,7 @@ inline int row_sm(const Layout& L) {
 }
 
 int prepare_lds(const void* kern, size_t lds);
+size_t widen_lds(const void* kern, size_t base, size_t want, long long grid);
 int tune_value(int key);
 int current_device_index();
 

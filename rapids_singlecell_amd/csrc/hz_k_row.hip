@@ -7,8 +7,11 @@ namespace hz {
 // ---------------------------------------------------------------- matrix-core dispatch
 
 template <int PREC, int NT, int SM>
-static int launch_rm(RowMArgs a, int grid, size_t lds, hipStream_t s) {
+static int launch_rm(const Layout& L, RowMArgs a, int grid, size_t lds, hipStream_t s) {
   auto kern = row_mfma_kernel<PREC, NT, SM>;
+  lds = widen_lds(reinterpret_cast<const void*>(kern), lds, block_update_full_lds<float>(L.BL, L.K),
+                  grid);
+  a.smem_bytes = (int)lds;
   HZ_TRY(prepare_lds(reinterpret_cast<const void*>(kern), lds));
   hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), lds, s, a);
   HZ_LAUNCHED("row_mfma_kernel");
@@ -19,15 +22,14 @@ static int launch_rm(RowMArgs a, int grid, size_t lds, hipStream_t s) {
 
 template <int PREC, int SM>
 static int launch_rm_sm(const Layout& L, RowMArgs a, int grid, hipStream_t s) {
-  const size_t lds = std::max(row_mfma_lds<PREC, SM>(L.Kp), (size_t)block_update_min_lds<float>(L.B));
+  const size_t lds = std::max(row_mfma_lds<PREC, SM>(L.Kp), (size_t)block_update_min_lds<float>(L.BL));
   HZ_RET_IF(lds > (size_t)device_lds_limit(), HZ_ERR_UNSUPPORTED,
             "row kernel needs %zu bytes of LDS (limit %d)", lds, device_lds_limit());
-  a.smem_bytes = (int)lds;
   switch (L.NT) {
-    case 4: return launch_rm<PREC, 4, SM>(a, grid, lds, s);
-    case 7: return launch_rm<PREC, 7, SM>(a, grid, lds, s);
-    case 8: return launch_rm<PREC, 8, SM>(a, grid, lds, s);
-    default: return launch_rm<PREC, 16, SM>(a, grid, lds, s);
+    case 4: return launch_rm<PREC, 4, SM>(L, a, grid, lds, s);
+    case 7: return launch_rm<PREC, 7, SM>(L, a, grid, lds, s);
+    case 8: return launch_rm<PREC, 8, SM>(L, a, grid, lds, s);
+    default: return launch_rm<PREC, 16, SM>(L, a, grid, lds, s);
   }
 }
 

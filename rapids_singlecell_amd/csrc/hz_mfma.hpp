@@ -286,7 +286,7 @@ __global__ __launch_bounds__(kBlock) void row_mfma_kernel(RowMArgs a) {
   // workgroup-level combine: consecutive waves of one segment share one add;
   // all adds are issued before any result is consumed
   {
-    const int rep = blockIdx.x % kRep;
+    const int rep = blockIdx.x % kAccRep, orep = blockIdx.x % kRep;
     long long* acc = a.acc_out + rep * a.epi.rep_stride;
     unsigned long long sink = 0;
     for (int k = tid; k < K; k += kBlock) {
@@ -311,8 +311,8 @@ __global__ __launch_bounds__(kBlock) void row_mfma_kernel(RowMArgs a) {
         km4 += s_obj2[w][0];
         ent4 += s_obj2[w][1];
       }
-      const unsigned long long r0 = fixed_add_wait(&a.epi.obj_acc[2 * rep], km4, a.scale_obj);
-      const unsigned long long r1 = fixed_add_wait(&a.epi.obj_acc[2 * rep + 1], ent4, a.scale_obj);
+      const unsigned long long r0 = fixed_add_wait(&a.epi.obj_acc[2 * orep], km4, a.scale_obj);
+      const unsigned long long r1 = fixed_add_wait(&a.epi.obj_acc[2 * orep + 1], ent4, a.scale_obj);
       sink += r0 + r1;
     }
     asm volatile("" ::"v"(sink));

@@ -36,12 +36,13 @@
 
 namespace hz {
 
-// replicas of the block accumulators written by the sweep (<= kRep).  One:
+// replicas of the block accumulators written by the sweep (<= kAccRep).  One:
 // every workgroup reads every entry of each block's sums, so replicas
 // multiply the bytes all workgroups pull through the fabric per block (the
 // adds themselves are fire-and-forget).  The two objective accumulators, to
 // which every workgroup adds once at the end, keep kRep replicas.
 constexpr int kSweepRep = 1;
+static_assert(kSweepRep <= kAccRep, "the workspace holds kAccRep replicas of acc_out");
 
 // ---------------------------------------------------------------- old-row sums
 
