@@ -21,7 +21,7 @@ HZ_PREC_NATIVE = 0
 HZ_PREC_SPLIT_BF16 = 1
 HZ_CORR = {"batched": 0, "fast": 1, "original": 2}
 TUNE_KEYS = {"row_chunk": 0, "centroid_wg": 1, "no_sweep": 2, "sweep_wg": 3, "sweep_plain": 4,
-             "spin_limit": 5}
+             "spin_limit": 5, "block_update": 6}
 PROF_KINDS = ["row", "centroid", "yreduce", "order", "correction", "phi", "apply", "sweep"]
 
 _vp = ctypes.c_void_p

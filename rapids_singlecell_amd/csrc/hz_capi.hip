@@ -419,6 +419,30 @@ static int launch_row_ct(const Layout& L, RowArgs<T> a, int grid, size_t lds, hi
   return HZ_OK;
 }
 
+// The update between block `block` and the next in its own launch, two
+// columns per workgroup (one round of loads per workgroup).  Used when the
+// levels x clusters entries are too many for a row launch's last workgroup.
+constexpr long long kExtUpdateEntries = 4096;
+
+static bool ext_update_of(const Layout& L) {
+  const int t = g_tune[HZ_TUNE_BLOCK_UPDATE];
+  if (t == 1) return false;
+  if (t == 2) return true;
+  return (long long)L.BL * L.K > kExtUpdateEntries;
+}
+
+template <typename T>
+static int launch_block_update(const Layout& L, const EpiArgs<T>& e, int block, hipStream_t s) {
+  constexpr int cols = 2;
+  const size_t lds = block_update_full_lds<T>(L.BL, cols);
+  auto kern = block_update_kernel<T>;
+  HZ_TRY(prepare_lds(reinterpret_cast<const void*>(kern), lds));
+  hipLaunchKernelGGL(kern, dim3((int)ceil_div(L.K, cols)), dim3(kBlock), lds, s, e, block, cols,
+                     (int)lds);
+  HZ_LAUNCHED("block_update_kernel");
+  return HZ_OK;
+}
+
 template <typename T>
 static int launch_row(const Layout& L, RowArgs<T> a, int grid, hipStream_t s) {
   const size_t lds = row_lds_bytes<T>(L);
@@ -687,7 +711,7 @@ static int init_impl(const hz_dims* dm, const T* Z, const T* Y0, const int* cats
   hipLaunchKernelGGL(centroids_to_yt_kernel<T>, dim3(1), dim3(kBlock), 0, s, Y0,
                      reinterpret_cast<T*>(L.YT), L.K, L.d, L.Kp);
   HZ_LAUNCHED("centroids_to_yt");
-  RowArgs<T> a;
+  RowArgs<T> a{};
   a.Z = Z;
   a.YT = reinterpret_cast<const T*>(L.YT);
   a.R = R;
@@ -861,7 +885,7 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
   ya.ycnt = L.ycnt;
   const dim3 ygrid(L.K, kYSlices);
 
-  RowArgs<T> ra;
+  RowArgs<T> ra{};
   ra.Z = Z;
   ra.YT = reinterpret_cast<const T*>(L.YT);
   ra.R = R;
@@ -1118,6 +1142,8 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
         HZ_LAUNCHED("yreduce_kernel");
         if (use_rm) HZ_TRY(launch_split(L, L.ctrl, s));
       }
+      const bool ext_upd = ext_update_of(L);
+      rm.ext_update = ra.ext_update = ext_upd ? 1 : 0;
       for (int j = 0; j < L.pd.nblk; ++j) {
         const int rows_j = std::min(L.bs, L.n - j * L.bs);
         const int grid = (int)(ceil_div(rows_j, L.chunk) + L.B);
@@ -1133,6 +1159,7 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
           ra.counter = L.counters + j;
           HZ_TRY(launch_row<T>(L, ra, grid, s));
         }
+        if (ext_upd && j + 1 < L.pd.nblk) HZ_TRY(launch_block_update<T>(L, epi, j, s));
       }
     }
   }

@@ -115,9 +115,12 @@ enum : int {
 // Block contributions arrive as fixed-point sums (acc_out / acc_in); columns
 // are processed in slabs that fit `smem_bytes` of LDS.  Every thread of the
 // workgroup must call it.  Returns the objective when kUpdObj/kUpdObjInit.
+// Only the columns [kbeg, kend) are updated (every entry's arithmetic
+// involves its own column only); the objective needs all of them.
 template <typename T>
 __device__ T block_update(const EpiArgs<T>& a, unsigned char* smem, int smem_bytes, int seg_out,
-                          int seg_in, int flags, double* red) {
+                          int seg_in, int flags, double* red, int kbeg = 0,
+                          int kend = 0x7fffffff) {
   const int B = a.B, K = a.K;
   const int tid = threadIdx.x, nt = blockDim.x;
   // Pr and theta per level, then the column slabs
@@ -138,8 +141,9 @@ __device__ T block_update(const EpiArgs<T>& a, unsigned char* smem, int smem_byt
   T* s_E = s_O + (size_t)B * KS;
   const double inv = a.inv_scale_o;
   double div = 0.0;
-  for (int k0 = 0; k0 < K; k0 += KS) {
-    const int ks = min(KS, K - k0);
+  const int kstop = min(K, kend);
+  for (int k0 = kbeg; k0 < kstop; k0 += KS) {
+    const int ks = min(KS, kstop - k0);
     const int n = B * ks;
     __syncthreads();
     // every load of kUpdU entries per thread (sum replicas, old sums, O, E)
@@ -272,6 +276,21 @@ template <typename T>
 __host__ __device__ inline size_t block_update_full_lds(int B, int K) {
   return (size_t)((2 * B * (int)sizeof(T) + 15) & ~15) +
          (size_t)K * (B * (16 + 2 * (int)sizeof(T)) + 16);
+}
+
+// The update between block `block` and the next (no objective term), for
+// the `cols` columns from blockIdx.x * cols: the same arithmetic per entry
+// as in a row launch's last workgroup, spread over K / cols workgroups so
+// that many levels do not make one workgroup's serial tail.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void block_update_kernel(EpiArgs<T> e, int block, int cols,
+                                                              int smem_bytes) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double red[kBlock / kWave];
+  if (e.ctrl->done) return;
+  const int k0 = blockIdx.x * cols;
+  block_update(e, smem, smem_bytes, block * e.B, (block + 1) * e.B, kUpdOut | kUpdIn | kUpdPen,
+               red, k0, k0 + cols);
 }
 
 // ---------------------------------------------------------------- arrival
@@ -617,6 +636,7 @@ struct RowArgs {
   int n, d, K, Kp;
   int vec_store;
   int smem_bytes;  // dynamic LDS size (reused by the epilogue)
+  int ext_update;  // 1: the update after a block other than the last is block_update_kernel's
   double scale_o, scale_obj;
   T* obj_out;  // initialisation: objective written here
   EpiArgs<T> epi;
@@ -779,6 +799,7 @@ __global__ __launch_bounds__(kBlock) void row_kernel(RowArgs<T> a) {
     }
     asm volatile("" ::"v"(sink));
   }
+  if (a.ext_update && a.block >= 0 && a.block + 1 < a.epi.nblk) return;
   if (!last_to_arrive(a.counter, &s_last)) return;
   const EpiArgs<T>& e = a.epi;
   if (a.block < 0) {

@@ -85,6 +85,7 @@ struct RowMArgs {
   float term;
   int n, d, K, Kp, dP;
   int smem_bytes;
+  int ext_update;  // 1: the update after a block other than the last is block_update_kernel's
   double scale_o, scale_obj;
   float* obj_out;
   EpiArgs<float> epi;
@@ -317,6 +318,7 @@ __global__ __launch_bounds__(kBlock) void row_mfma_kernel(RowMArgs a) {
     }
     asm volatile("" ::"v"(sink));
   }
+  if (a.ext_update && a.block >= 0 && a.block + 1 < a.epi.nblk) return;
   if (!last_to_arrive(a.counter, &s_last)) return;
   const EpiArgs<float>& e = a.epi;
   if (a.block < 0) {

@@ -402,6 +402,28 @@ def test_clustering_sweep_vs_per_block(device, tuning, precision, stabilized):
     assert rel(out[0][0], out[1][0]) < 1e-4
 
 
+@pytest.mark.parametrize("dtype,precision", [(np.float32, "native"), (np.float32, "split_bf16"),
+                                             (np.float64, "native")])
+def test_block_update_launch_is_bitwise_the_same(device, tuning, dtype, precision):
+    """One launch per block: the O/E/penalty update between blocks runs in the
+    row launch's last workgroup or in a launch of its own spread over the
+    clusters (the automatic choice above 4096 levels x clusters, forced here).
+    Every entry's arithmetic is the same, so the results are the same bits."""
+    n, d, K, B = 20000, 20, 30, 6
+    X, obs = synthetic_pcs(n, d, 6, B, seed=7)
+    Xt = torch.as_tensor(X.astype(dtype), device=device)
+    Zn = X / np.linalg.norm(X, axis=1, keepdims=True)
+    Y0 = Zn[np.random.default_rng(7).choice(n, K, replace=False)].astype(dtype)
+    out = {}
+    for mode in (1, 2):
+        tuning(no_sweep=1, block_update=mode)
+        Z = harmonize(Xt, obs, "batch", n_clusters=K, init_centroids=Y0, random_state=0,
+                      max_iter_harmony=2, max_iter_clustering=5, precision=precision)
+        out[mode] = Z.cpu().numpy()
+    assert np.isfinite(out[1]).all()
+    np.testing.assert_array_equal(out[2], out[1])
+
+
 @pytest.mark.parametrize("precision", ["native", "split_bf16"])
 @pytest.mark.parametrize("K,d", [(50, 50), (75, 30), (64, 17), (33, 64), (130, 50)])
 def test_harmonize_cluster_counts(device, precision, K, d):

@@ -18,6 +18,7 @@ import pytest
 import torch
 
 from oracle import harmony_oracle as ho
+from rapids_singlecell_amd import _native
 from rapids_singlecell_amd.preprocessing._harmony import harmonize
 
 pytestmark = pytest.mark.gpu
@@ -108,3 +109,34 @@ def test_multi_duplicated_covariate_equals_one_covariate(device):
           flush=True)
     np.testing.assert_allclose(r2["objectives"], r1["objectives"], rtol=1e-10)
     assert e < 1e-9
+
+
+@pytest.mark.parametrize("dtype,precision", [(np.float64, "native"), (np.float32, "native"),
+                                             (np.float32, "split_bf16")])
+def test_multi_update_launch_is_bitwise_the_same(device, dtype, precision):
+    """80 levels x 60 clusters (4800 entries): the O/E/penalty update between
+    blocks runs in a launch of its own spread over the clusters, the automatic
+    choice above 4096 entries.  Forcing it into the row launch's last
+    workgroup gives the same bits; the FP64 run also matches the oracle."""
+    X, obs, b, dn, Y0 = _data(20000, 20, 16, 64, 60, seed=6)
+    kw = dict(n_clusters=60, max_iter_harmony=2, max_iter_clustering=6, random_state=3,
+              stabilized_penalty=False, dynamic_lambda=False, ridge_lambda=1.0, **FIXED)
+    before = _native.get_tuning()
+    res = {}
+    try:
+        for mode in (0, 1, 2):  # automatic, last workgroup, own launch
+            _native.set_tuning(block_update=mode)
+            res[mode] = _gpu(X, obs, ["batch", "donor"], Y0, dtype, multi_covariate=True,
+                             precision=precision, **kw)
+    finally:
+        _native.set_tuning(**before)
+    np.testing.assert_array_equal(res[2][0], res[1][0])
+    np.testing.assert_array_equal(res[0][0], res[2][0])
+    np.testing.assert_array_equal(np.asarray(res[2][1]["objectives"]),
+                                  np.asarray(res[1][1]["objectives"]))
+    if dtype == np.float64:
+        Zr = ho.harmonize_multi(X, [b, dn], [16, 64], Y0=Y0, **kw)
+        e = _rel(res[2][0], Zr)
+        print(f"\nmulti-covariate fp64, 80 levels x 60 clusters, update in its own launch: "
+              f"rel err Z_corr vs oracle = {e:.3e}", flush=True)
+        assert e < 1e-8
