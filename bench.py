@@ -674,7 +674,8 @@ def main() -> int:
                     help="time the brute-force kNN of the neighbour graph")
     ap.add_argument("--knn-sizes", type=int, nargs="+", default=[200_000, 1_300_000])
     ap.add_argument("--tune", action="append", default=[],
-                    help="library tuning knob key=value (row_chunk, centroid_wg, no_sweep, sweep_wg)")
+                    help="library tuning knob key=value (row_chunk, centroid_wg, no_sweep, sweep_wg, "
+                         "sweep_plain, spin_limit, block_update)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the Harmony path "

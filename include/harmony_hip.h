@@ -116,9 +116,10 @@ int hz_profile_collect(double* ms_out, long long* launches_out, int n_kinds);
 #define HZ_TUNE_SPIN_LIMIT 5 /* polls a block-sweep workgroup makes while waiting for the
                                  others' block sums before it gives up and the call returns
                                  HZ_ERR_LAUNCH (0 = 65536, about 65 ms) */
-#define HZ_TUNE_BLOCK_UPDATE 6 /* per-block path, update between blocks: 0 = automatic, 1 = in the
-                                  row launch's last workgroup, 2 = a separate launch spread over
-                                  the clusters (identical results) */
+#define HZ_TUNE_BLOCK_UPDATE 6 /* per-block path, update between blocks: 0 = automatic (a separate
+                                  launch above 512 levels x clusters), 1 = in the row launch's
+                                  last workgroup, 2 = a separate launch spread over the clusters
+                                  (identical results) */
 #define HZ_TUNE_KEYS 8
 int hz_set_tuning(int key, int value);
 int hz_get_tuning(int key);

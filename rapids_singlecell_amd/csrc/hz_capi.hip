@@ -420,9 +420,11 @@ static int launch_row_ct(const Layout& L, RowArgs<T> a, int grid, size_t lds, hi
 }
 
 // The update between block `block` and the next in its own launch, two
-// columns per workgroup (one round of loads per workgroup).  Used when the
-// levels x clusters entries are too many for a row launch's last workgroup.
-constexpr long long kExtUpdateEntries = 4096;
+// columns per workgroup (one round of loads per workgroup).  Used above 512
+// levels x clusters entries: beyond one round of loads in the row launch's
+// last workgroup, its serial tail costs more than the extra launch (at 16 x
+// 100 the per-block rate rises 6% FP32, 2% FP64 at 1.3M cells).
+constexpr long long kExtUpdateEntries = 512;
 
 static bool ext_update_of(const Layout& L) {
   const int t = g_tune[HZ_TUNE_BLOCK_UPDATE];

@@ -213,14 +213,16 @@ static __global__ __launch_bounds__(256) void perm_keys_lds_kernel(int n, uint32
 
 // Exclusive scan of `n` ints by one workgroup of kScanThreads threads, in
 // coalesced tiles of kScanThreads * kScanPer elements with a running carry.
-// out[0..n] (out[n] = total).  Optional second output `out2[0..n-1]`.
+// out[0..n] (out[n] = total, unless `total` is false), starting from
+// `carry0`.  Optional second output `out2[0..n-1]`.
 // `lds` holds at least kScanThreads / 64 ints.
 static __device__ void wg_exclusive_scan(const int* __restrict__ in, int* __restrict__ out,
-                                  int* __restrict__ out2, long long n, int* lds) {
+                                  int* __restrict__ out2, long long n, int* lds, int carry0 = 0,
+                                  bool total = true) {
   constexpr int kWaves = kScanThreads / 64;
   constexpr int kTile = kScanThreads * kScanPer;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  int carry = 0;
+  int carry = carry0;
   for (long long base = 0; base < n; base += kTile) {
     const long long i0 = base + (long long)t * kScanPer;
     int v[kScanPer];
@@ -257,7 +259,7 @@ static __device__ void wg_exclusive_scan(const int* __restrict__ in, int* __rest
     }
     carry += tot;
   }
-  if (t == 0) out[n] = carry;
+  if (total && t == 0) out[n] = carry;
   __syncthreads();
 }
 
@@ -405,6 +407,27 @@ static __global__ __launch_bounds__(64) void group_place_kernel(
       active &= ~m;
     }
   }
+}
+
+// Per-tile offsets of a key-sorted order's (block, batch, tile) counts, one
+// workgroup per block: block j holds the sorted positions [j bs, (j+1) bs)
+// (the last one fewer), so its counts scan on their own from j * bs.  Also
+// the segment starts (seg_start[nseg] = n); chunk_table_kernel without
+// `thist` then counts and scans the chunks.
+static __global__ __launch_bounds__(kScanThreads) void block_tile_scan_kernel(
+    const int* __restrict__ done, const int* __restrict__ thist, int* __restrict__ tbase,
+    int nblk, int nb, int tpb, int bs, int n, int* __restrict__ seg_start, PermStride st) {
+  __shared__ int lds[kScanThreads];
+  if (done && *done) return;
+  const int y = blockIdx.y, j = blockIdx.x;
+  thist += st.th * y;
+  tbase += st.th * y;
+  seg_start += st.seg * y;
+  const long long ne = (long long)nb * tpb, e0 = (long long)j * ne;
+  wg_exclusive_scan(thist + e0, tbase + e0, nullptr, ne, lds, j * bs, j == nblk - 1);
+  for (int b = threadIdx.x; b < nb; b += blockDim.x)
+    seg_start[(long long)j * nb + b] = tbase[e0 + (long long)b * tpb];
+  if (j == nblk - 1 && threadIdx.x == 0) seg_start[(long long)nblk * nb] = n;
 }
 
 // Segment starts and first chunk of each segment from per-tile offsets (one
@@ -933,8 +956,11 @@ static int launch_block_order(const PermDims& p, uint32_t seed, const int* cats,
   const size_t lds = (size_t)p.nb * sizeof(int);
   const int tiles = p.nblk * p.tpb;
   const PermStride st = perm_stride(p);
-  hipLaunchKernelGGL(chunk_table_kernel, dim3(1, I), dim3(kScanThreads), 0, s, done, b.thist,
-                     b.tbase, nseg, p.tpb, p.n, p.chunk, t.seg_start, t.seg_ch, t.ch_start,
+  hipLaunchKernelGGL(block_tile_scan_kernel, dim3(p.nblk, I), dim3(kScanThreads), 0, s, done,
+                     b.thist, b.tbase, p.nblk, p.nb, p.tpb, p.bs, p.n, t.seg_start, st);
+  HZ_LAUNCHED("block_tile_scan");
+  hipLaunchKernelGGL(chunk_table_kernel, dim3(1, I), dim3(kScanThreads), 0, s, done, nullptr,
+                     nullptr, nseg, p.tpb, p.n, p.chunk, t.seg_start, t.seg_ch, t.ch_start,
                      t.ch_len, t.ch_seg, nch_tmp, st);
   HZ_LAUNCHED("chunk_table");
   hipLaunchKernelGGL(chunk_fill_kernel, dim3(launch_grid_1d(p.maxch, I > 1 ? 2 : 8), I),

@@ -407,7 +407,7 @@ def test_clustering_sweep_vs_per_block(device, tuning, precision, stabilized):
 def test_block_update_launch_is_bitwise_the_same(device, tuning, dtype, precision):
     """One launch per block: the O/E/penalty update between blocks runs in the
     row launch's last workgroup or in a launch of its own spread over the
-    clusters (the automatic choice above 4096 levels x clusters, forced here).
+    clusters (the automatic choice above 512 levels x clusters, forced here).
     Every entry's arithmetic is the same, so the results are the same bits."""
     n, d, K, B = 20000, 20, 30, 6
     X, obs = synthetic_pcs(n, d, 6, B, seed=7)

@@ -116,7 +116,7 @@ def test_multi_duplicated_covariate_equals_one_covariate(device):
 def test_multi_update_launch_is_bitwise_the_same(device, dtype, precision):
     """80 levels x 60 clusters (4800 entries): the O/E/penalty update between
     blocks runs in a launch of its own spread over the clusters, the automatic
-    choice above 4096 entries.  Forcing it into the row launch's last
+    choice above 512 entries.  Forcing it into the row launch's last
     workgroup gives the same bits; the FP64 run also matches the oracle."""
     X, obs, b, dn, Y0 = _data(20000, 20, 16, 64, 60, seed=6)
     kw = dict(n_clusters=60, max_iter_harmony=2, max_iter_clustering=6, random_state=3,
