@@ -395,7 +395,6 @@ __global__ __launch_bounds__(kBlock) void centroid_kernel(CentArgs<T> a) {
   const int c1 = min(nch, c0 + a.cpw);
   auto ch_start = [&](int c) { return natural ? 64 * c : a.ch_start[c]; };
   auto ch_len = [&](int c) { return natural ? min(64, a.n - 64 * c) : a.ch_len[c]; };
-  auto cell_at = [&](int p) -> long long { return natural ? (long long)p : (long long)a.order[p]; };
 
   T acc[KT][DT];
 #pragma unroll
@@ -404,30 +403,44 @@ __global__ __launch_bounds__(kBlock) void centroid_kernel(CentArgs<T> a) {
     for (int j = 0; j < DT; ++j) acc[i][j] = T(0);
 
   T rv[MR], zv[MZ];
-  // tile cursor: chunk c, row offset r0 inside the chunk
+  // tile cursor: chunk c, row offset r0 inside the chunk.  Every load is
+  // issued from a valid address (the tile's first row, column 0, in place of
+  // a padding entry) and masked afterwards, and a grouped order's cell ids
+  // are all requested before the data: without a branch per element the
+  // tile's loads are in flight together instead of one wait per element.
   auto fetch = [&](int c, int r0) {
     const int p0 = ch_start(c) + r0;
     const int rows = min(TR, ch_len(c) - r0);
+    int cr[MR], cz[MZ];  // position, then cell, of each element's row
 #pragma unroll
     for (int m = 0; m < MR; ++m) {
-      const int e = tid + m * kBlock;
-      const int r = e / Kp, k = e % Kp;
-      T v = T(0);
-      if (r < rows && k < K) v = a.R[cell_at(p0 + r) * K + k];
-      rv[m] = v;
+      const int e = tid + m * kBlock, r = e / Kp, k = e % Kp;
+      cr[m] = p0 + ((r < rows && k < K) ? r : 0);
     }
 #pragma unroll
     for (int m = 0; m < MZ; ++m) {
-      const int e = tid + m * kBlock;
-      const int r = e / Dp, cc = e % Dp;
-      T v = T(0);
-      if (r < rows) {
-        if (cc < d)
-          v = a.Z[cell_at(p0 + r) * d + cc];
-        else if (cc == d)
-          v = T(1);
-      }
-      zv[m] = v;
+      const int e = tid + m * kBlock, r = e / Dp, cc = e % Dp;
+      cz[m] = p0 + ((r < rows && cc < d) ? r : 0);
+    }
+    if (!natural) {
+#pragma unroll
+      for (int m = 0; m < MR; ++m) cr[m] = a.order[cr[m]];
+#pragma unroll
+      for (int m = 0; m < MZ; ++m) cz[m] = a.order[cz[m]];
+    }
+#pragma unroll
+    for (int m = 0; m < MR; ++m) {
+      const int e = tid + m * kBlock, r = e / Kp, k = e % Kp;
+      const bool ok = r < rows && k < K;
+      const T v = a.R[(long long)cr[m] * K + (ok ? k : 0)];
+      rv[m] = ok ? v : T(0);
+    }
+#pragma unroll
+    for (int m = 0; m < MZ; ++m) {
+      const int e = tid + m * kBlock, r = e / Dp, cc = e % Dp;
+      const bool ok = r < rows && cc < d;
+      const T v = a.Z[(long long)cz[m] * d + (ok ? cc : 0)];
+      zv[m] = ok ? v : ((r < rows && cc == d) ? T(1) : T(0));
     }
   };
 
