@@ -705,10 +705,26 @@ __global__ __launch_bounds__(kBlock) void row_kernel(RowArgs<T> a) {
       __syncthreads();
       if (tid < kTileRows) idx_s[tid] = (tid < rows) ? a.order[p0 + r0 + tid] : -1;
       __syncthreads();
-      for (int e = tid; e < kTileRows * d; e += kBlock) {
-        const int r = e / d, cc = e - r * d;
-        const int i = idx_s[r];
-        ZTs[cc * kZtStride + r] = (i >= 0) ? a.Z[(long long)i * d + cc] : T(0);
+      // kZU loads per thread in flight together (row 0's cell stands in for
+      // rows past the chunk; those entries are stored as 0)
+      constexpr int kZU = 8;
+      for (int e0 = tid; e0 < kTileRows * d; e0 += kZU * kBlock) {
+        T zv[kZU];
+#pragma unroll
+        for (int u = 0; u < kZU; ++u) {
+          const int e = min(e0 + u * kBlock, kTileRows * d - 1);
+          const int r = e / d, cc = e - r * d;
+          const int i = idx_s[r];
+          zv[u] = a.Z[(long long)(i >= 0 ? i : idx_s[0]) * d + cc];
+        }
+#pragma unroll
+        for (int u = 0; u < kZU; ++u) {
+          const int e = e0 + u * kBlock;
+          if (e < kTileRows * d) {
+            const int r = e / d, cc = e - r * d;
+            ZTs[cc * kZtStride + r] = (idx_s[r] >= 0) ? zv[u] : T(0);
+          }
+        }
       }
       __syncthreads();
       T acc[4][CT];
