@@ -1128,9 +1128,12 @@ static int clustering_impl(const hz_dims* dm, const T* Z, T* R, T* E, T* O, cons
         } else if (use_cm) {
           HZ_TRY(launch_cent_mfma(L, cm, L.Gm, s));
         } else if (blk_natural) {
+          // natural order: ceil(n / 64) chunks of 64 cells over the L.G
+          // workgroups (the grouped table's count would leave some idle)
           CentArgs<T> can = ca;
           can.order = nullptr;
           can.acc_in = nullptr;
+          can.cpw = (int)ceil_div(ceil_div((long long)L.n, 64), (long long)L.G);
           HZ_TRY(launch_centroid<T>(L, can, L.G, s));
         } else {
           HZ_TRY(launch_centroid<T>(L, ca, L.G, s));
