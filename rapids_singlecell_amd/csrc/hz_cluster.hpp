@@ -682,19 +682,38 @@ __global__ __launch_bounds__(kBlock) void row_kernel(RowArgs<T> a) {
     const int seg = a.ch_seg[c];
     const int b = seg - a.seg_base;
     const int p0 = a.ch_start[c], len = a.ch_len[c];
-    // normalised centroids: d x Kp contiguous, 16-byte vectors
+    // normalised centroids: d x Kp contiguous, 16-byte vectors, eight loads
+    // per thread in flight together (from valid addresses)
     {
-      constexpr int V = 16 / sizeof(T);
+      constexpr int V = 16 / sizeof(T), kU = 8;
       using VT = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
       const VT* src = reinterpret_cast<const VT*>(a.YT);
       VT* dst = reinterpret_cast<VT*>(YTs);
-      for (int e = tid; e < d * Kp / V; e += kBlock) dst[e] = src[e];
+      const int nv = d * Kp / V;
+      for (int e0 = tid; e0 < nv; e0 += kU * kBlock) {
+        VT t[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) t[u] = src[min(e0 + u * kBlock, nv - 1)];
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+          if (e0 + u * kBlock < nv) dst[e0 + u * kBlock] = t[u];
+      }
     }
     T pv[CT];
+    if (a.pen && !a.epi.lv.glev) {
+      // one covariate: the loads from valid addresses, in flight together
 #pragma unroll
-    for (int q = 0; q < CT; ++q) {
-      const int kk = cg * CT + q;
-      pv[q] = (kk < K) ? (a.pen ? group_pen(a.pen, a.epi.lv, b, kk, K) : T(1)) : T(0);
+      for (int q = 0; q < CT; ++q) {
+        const int kk = cg * CT + q;
+        const T v = a.pen[(long long)b * K + min(kk, K - 1)];
+        pv[q] = (kk < K) ? v : T(0);
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < CT; ++q) {
+        const int kk = cg * CT + q;
+        pv[q] = (kk < K) ? (a.pen ? group_pen(a.pen, a.epi.lv, b, kk, K) : T(1)) : T(0);
+      }
     }
     double km = 0.0, ent = 0.0;
     double dO[CT];
@@ -707,7 +726,7 @@ __global__ __launch_bounds__(kBlock) void row_kernel(RowArgs<T> a) {
       __syncthreads();
       // kZU loads per thread in flight together (row 0's cell stands in for
       // rows past the chunk; those entries are stored as 0)
-      constexpr int kZU = 8;
+      constexpr int kZU = 4;
       for (int e0 = tid; e0 < kTileRows * d; e0 += kZU * kBlock) {
         T zv[kZU];
 #pragma unroll

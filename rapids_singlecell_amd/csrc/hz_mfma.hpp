@@ -102,6 +102,56 @@ __host__ __device__ inline size_t row_mfma_lds(int Kp) {
   return (PREC == kPrecF32) ? (size_t)4 * SM * (Kp + 16) * 4 : (size_t)Kp * (32 * SM + 8) * 2 * 2;
 }
 
+// Copies the normalised centroids into LDS in the padded layouts above
+// (f32: 4*SM rows of YT, rows d.. zero; bf16: YH, then YL), eight 16-byte
+// loads per thread in flight together: every load comes from a valid
+// address and only entries inside the layout are stored.
+template <int PREC, int SM>
+__device__ __forceinline__ void stage_centroids(const float* YT, const __bf16* YH,
+                                                const __bf16* YL, unsigned char* smem, int d,
+                                                int Kp, int dP, int tid) {
+  constexpr int kU = 8;
+  if constexpr (PREC == kPrecF32) {
+    const float4* src = reinterpret_cast<const float4*>(YT);
+    float4* dst = reinterpret_cast<float4*>(smem);
+    const int per = Kp / 4, perS = (Kp + 16) / 4, tot = 4 * SM * per, nsrc = d * per;
+    for (int e0 = tid; e0 < tot; e0 += kU * kBlock) {
+      float4 t[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) t[u] = src[min(e0 + u * kBlock, nsrc - 1)];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int e = e0 + u * kBlock;
+        if (e < tot) dst[(e / per) * perS + e % per] = (e < nsrc) ? t[u] : float4{0.f, 0.f, 0.f, 0.f};
+      }
+    }
+  } else {
+    constexpr int kH = kU / 2;
+    const uint4* sh = reinterpret_cast<const uint4*>(YH);
+    const uint4* sl = reinterpret_cast<const uint4*>(YL);
+    uint4* dst = reinterpret_cast<uint4*>(smem);
+    const int per = dP / 8, perS = (dP + 8) / 8, tot = Kp * per, half = Kp * perS;
+    for (int e0 = tid; e0 < tot; e0 += kH * kBlock) {
+      uint4 th[kH], tl[kH];
+#pragma unroll
+      for (int u = 0; u < kH; ++u) {
+        const int e = min(e0 + u * kBlock, tot - 1);
+        th[u] = sh[e];
+        tl[u] = sl[e];
+      }
+#pragma unroll
+      for (int u = 0; u < kH; ++u) {
+        const int e = e0 + u * kBlock;
+        if (e < tot) {
+          const int o = (e / per) * perS + e % per;
+          dst[o] = th[u];
+          dst[half + o] = tl[u];
+        }
+      }
+    }
+  }
+}
+
 // SM = number of MFMA k-steps over the d columns (4 per step for f32, 32 for
 // bf16), a compile-time constant so the matrix loop is straight-line code.
 template <int PREC, int NT, int SM>
@@ -153,38 +203,27 @@ __global__ __launch_bounds__(kBlock) void row_mfma_kernel(RowMArgs a) {
   // stage the normalised centroids (row strides padded against bank conflicts)
   const int KpS = Kp + 16;      // f32: floats per LDS row of YT
   const int dPS = a.dP + 8;     // bf16: elements per LDS row of YH / YL
-  if (any) {
-    if (PREC == kPrecF32) {
-      // rows d .. 4*SM-1 are zero so the k-loop needs no bounds test
-      const float4* src = reinterpret_cast<const float4*>(a.YT);
-      float4* dst = reinterpret_cast<float4*>(smem);
-      const int per = Kp / 4, perS = KpS / 4;
-      for (int e = tid; e < 4 * SM * per; e += kBlock) {
-        const int row = e / per;
-        dst[row * perS + e % per] = (row < d) ? src[e] : float4{0.f, 0.f, 0.f, 0.f};
-      }
-    } else {
-      const uint4* sh = reinterpret_cast<const uint4*>(a.YH);
-      const uint4* sl = reinterpret_cast<const uint4*>(a.YL);
-      uint4* dst = reinterpret_cast<uint4*>(smem);
-      const int per = a.dP / 8, perS = dPS / 8;
-      const int half = Kp * perS;
-      for (int e = tid; e < Kp * per; e += kBlock) {
-        const int o = (e / per) * perS + e % per;
-        dst[o] = sh[e];
-        dst[half + o] = sl[e];
-      }
-    }
-  }
+  // (f32: rows d .. 4*SM-1 are zero so the k-loop needs no bounds test)
+  if (any) stage_centroids<PREC, SM>(a.YT, a.YH, a.YL, smem, d, Kp, a.dP, tid);
   __syncthreads();
 
   if (has) {
     const int b = seg - a.seg_base;
     float pv[NT];
+    if (a.pen && !a.epi.lv.glev) {
+      // one covariate: the loads from valid addresses, in flight together
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      const int k = 16 * t + lr;
-      pv[t] = (k < K) ? (a.pen ? group_pen(a.pen, a.epi.lv, b, k, K) : 1.f) : 0.f;
+      for (int t = 0; t < NT; ++t) {
+        const int k = 16 * t + lr;
+        const float v = a.pen[(long long)b * K + min(k, K - 1)];
+        pv[t] = (k < K) ? v : 0.f;
+      }
+    } else {
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const int k = 16 * t + lr;
+        pv[t] = (k < K) ? (a.pen ? group_pen(a.pen, a.epi.lv, b, k, K) : 1.f) : 0.f;
+      }
     }
     float dO[NT];
 #pragma unroll

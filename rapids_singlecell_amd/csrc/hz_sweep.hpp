@@ -270,26 +270,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& a, const int wg, con
   // stage the normalised centroids (same padded layouts as the row kernel)
   const int KpS = Kp + 16;
   const int dPS = a.dP + 8;
-  if (PREC == kPrecF32) {
-    const float4* src = reinterpret_cast<const float4*>(a.YT);
-    float4* dst = reinterpret_cast<float4*>(smem);
-    const int per = Kp / 4, perS = KpS / 4;
-    for (int e = tid; e < 4 * SM * per; e += kBlock) {
-      const int row = e / per;
-      dst[row * perS + e % per] = (row < d) ? src[e] : float4{0.f, 0.f, 0.f, 0.f};
-    }
-  } else {
-    const uint4* sh = reinterpret_cast<const uint4*>(a.YH);
-    const uint4* sl = reinterpret_cast<const uint4*>(a.YL);
-    uint4* dst = reinterpret_cast<uint4*>(smem);
-    const int per = a.dP / 8, perS = dPS / 8;
-    const int half = Kp * perS;
-    for (int e = tid; e < Kp * per; e += kBlock) {
-      const int o = (e / per) * perS + e % per;
-      dst[o] = sh[e];
-      dst[half + o] = sl[e];
-    }
-  }
+  stage_centroids<PREC, SM>(a.YT, a.YH, a.YL, smem, d, Kp, a.dP, tid);
   for (int q = tid; q < BK; q += kBlock) {
     s_O[q] = a.O[q];
     s_E[q] = a.E[q];
