@@ -176,24 +176,31 @@ def _c3_job(X, codes, B, K, precision):
 
 
 def test_fp64_large_blocks_cover_every_cell(device):
-    """FP64 (vector kernels) with blocks of more than 32768 cells: the row
-    kernel takes 128-row chunks while the centroid pass walks the cells in
-    natural 64-row pieces over the same workgroups, and every piece must be
+    """FP64 (vector kernels) with 128-row chunks (set here; the default is
+    64): the centroid pass walks the cells in natural 64-row pieces over
+    workgroups sized for the (fewer) grouped chunks, and every piece must be
     covered.  FP64 and FP32 (matrix-core path) R after 20 clustering
-    iterations at 700k cells agree to 1e-4; centroids from part of the cells
-    would move R by about 1e-2."""
+    iterations at 700k cells (blocks of 35000) agree to 1e-4; centroids from
+    part of the cells would move R by about 1e-2."""
+    from rapids_singlecell_amd import _native
+
     n, d, K, B = 700_000, 20, 16, 4
     X, obs = synthetic_pcs(n, d, 8, B, seed=1)
     codes = np.asarray(obs["batch"].cat.codes.values, dtype=np.int32)
     Z = ho.l2_normalize(X.astype(np.float64))
     Y0 = Z[np.random.default_rng(3).choice(n, K, replace=False)]
     Rs = {}
-    for name, Xs in (("fp64", X.astype(np.float64)), ("fp32", X)):
-        job = _c3_job(Xs, codes, B, K, 0)
-        job.init(torch.as_tensor(Y0.astype(Xs.dtype)))
-        assert job.clustering(0) == 20
-        Rs[name] = job.R.double().cpu().numpy()
-        del job
+    before = _native.get_tuning()
+    try:
+        for name, Xs in (("fp64", X.astype(np.float64)), ("fp32", X)):
+            _native.set_tuning(row_chunk=128 if name == "fp64" else 0)
+            job = _c3_job(Xs, codes, B, K, 0)
+            job.init(torch.as_tensor(Y0.astype(Xs.dtype)))
+            assert job.clustering(0) == 20
+            Rs[name] = job.R.double().cpu().numpy()
+            del job
+    finally:
+        _native.set_tuning(**before)
     e = rel(Rs["fp32"], Rs["fp64"])
     print(f"\n700k cells, blocks of 35000: R rel diff FP32 vs FP64 = {e:.2e}", flush=True)
     assert e < 1e-4
