@@ -417,6 +417,9 @@ def multi_bench(args) -> int:
     from rapids_singlecell_amd.preprocessing._harmony._job import HarmonyJob
 
     _native.load()
+    tune = {k: int(v) for k, v in (t.split("=", 1) for t in args.tune)}
+    if tune:
+        _native.set_tuning(**tune)
     n, d, K = args.multi_cells, 50, 100
     rng = np.random.default_rng(0)
     centers = rng.normal(0, 3.0, size=(50, d))

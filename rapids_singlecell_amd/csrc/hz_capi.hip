@@ -247,7 +247,10 @@ static int make_layout(const hz_dims* dm, void* base, Layout& L) {
   if (g_tune[HZ_TUNE_ROW_CHUNK] > 0)
     L.chunk = g_tune[HZ_TUNE_ROW_CHUNK];
   else if (L.mfma)
-    L.chunk = 16;  // one MFMA tile per chunk
+    // one MFMA tile per chunk for the block sweep; several covariates never
+    // take the sweep and run one wave's 64 rows per chunk (46.7 against
+    // 48.7 ms per FP32 Harmony iteration at 1.3M cells, 16 + 64 levels)
+    L.chunk = (L.C >= 2) ? 64 : 16;
   else
     L.chunk = 64;  // one LDS tile per workgroup (FP64 at 1.3M cells: 106 us per
                    // row launch, against 125 us with 128-row chunks)
