@@ -183,6 +183,17 @@ bool sweep_broken() { return g_sweep_broken[current_device_index()]; }
 
 // Layout is declared in hz_host.hpp
 
+// Shape rules of the block sweep that the layout fixes: at most 256
+// clusters and kSweepMaxBlocks blocks, one covariate, at least 4 PCs, R and Z
+// small enough for the sweep's 32-bit buffer offsets (n K 4, n d 4 bytes <=
+// kNoRow), and the sweep not switched off.
+static bool sweep_shape_ok(const Layout& L) {
+  return L.K <= 256 && ceil_div(L.n, L.bs) <= kSweepMaxBlocks && L.C == 1 && L.d >= 4 &&
+         (unsigned long long)L.n * (unsigned long long)L.K * 4ull <= kNoRow &&
+         (unsigned long long)L.n * (unsigned long long)L.d * 4ull <= kNoRow &&
+         g_tune[HZ_TUNE_SWEEP] == 0;
+}
+
 static int make_layout(const hz_dims* dm, void* base, Layout& L) {
   HZ_RET_IF(!dm, HZ_ERR_ARG, "dims is NULL");
   HZ_RET_IF(dm->dtype != HZ_F32 && dm->dtype != HZ_F64, HZ_ERR_ARG, "dtype must be 4 or 8");
@@ -247,10 +258,12 @@ static int make_layout(const hz_dims* dm, void* base, Layout& L) {
   if (g_tune[HZ_TUNE_ROW_CHUNK] > 0)
     L.chunk = g_tune[HZ_TUNE_ROW_CHUNK];
   else if (L.mfma)
-    // one MFMA tile per chunk for the block sweep; several covariates never
-    // take the sweep and run one wave's 64 rows per chunk (46.7 against
-    // 48.7 ms per FP32 Harmony iteration at 1.3M cells, 16 + 64 levels)
-    L.chunk = (L.C >= 2) ? 64 : 16;
+    // one MFMA tile per chunk for the block sweep; where the sweep cannot
+    // run, wider chunks: one wave's 64 rows for several covariates (46.7
+    // against 48.7 ms per FP32 Harmony iteration at 1.3M cells, 16 + 64
+    // levels), 32 rows for one covariate (one launch per block at 1.3M
+    // cells, 16 batches: 45.2 against 40.5 M cells-iters/s FP32)
+    L.chunk = (L.C >= 2) ? 64 : sweep_shape_ok(L) ? 16 : 32;
   else
     L.chunk = 64;  // one LDS tile per workgroup (FP64 at 1.3M cells: 106 us per
                    // row launch, against 125 us with 128-row chunks)
@@ -825,14 +838,10 @@ static void bind_sweep_tab(SweepArgs& sw, const ChunkTab& t) {
   sw.ch_seg = t.ch_seg;
 }
 
-// Shape rules of the block sweep: 16-row chunks, at most 256 clusters and
-// kSweepMaxBlocks blocks, one covariate, at least 4 PCs, and R and Z small
-// enough for the sweep's 32-bit buffer offsets (n K 4, n d 4 bytes <= kNoRow).
+// Rules of the block sweep: 16-row chunks, the shape rules above, and no
+// earlier sweep that failed on this device.
 static bool sweep_rules(const Layout& L) {
-  return L.chunk == 16 && L.K <= 256 && L.pd.nblk <= kSweepMaxBlocks && L.C == 1 && L.d >= 4 &&
-         (unsigned long long)L.n * (unsigned long long)L.K * 4ull <= kNoRow &&
-         (unsigned long long)L.n * (unsigned long long)L.d * 4ull <= kNoRow &&
-         g_tune[HZ_TUNE_SWEEP] == 0 && !sweep_broken();
+  return L.chunk == 16 && L.pd.nblk <= kSweepMaxBlocks && sweep_shape_ok(L) && !sweep_broken();
 }
 
 template <typename T>

@@ -364,8 +364,6 @@ def test_clustering_sweep_vs_per_block(device, tuning, precision, stabilized):
     pb = Problem(n=12000, d=30, B=5, K=24, n_types=8, dtype=np.float32, dirichlet=True)
     lib = _native.load()
     dm = _native.dims(pb.n, pb.d, pb.K, pb.B, int(pb.n * 0.05), 4, precision)
-    nb = lib.hz_workspace_bytes(ctypes.byref(dm))
-    assert nb > 0, lib.hz_last_error()
     Z64 = pb.Z_norm.astype(np.float64)
     Rr, Er, Or, _ = ho.init_clusters(Z64, pb.Y0.astype(np.float64), sigma=0.1,
                                      Pr_b=pb.Pr_b.astype(np.float64),
@@ -375,6 +373,9 @@ def test_clustering_sweep_vs_per_block(device, tuning, precision, stabilized):
     out = {}
     for no_sweep in (0, 1):
         tuning(no_sweep=no_sweep)
+        # the layout (chunk size) follows the tuning: size the workspace under it
+        nb = lib.hz_workspace_bytes(ctypes.byref(dm))
+        assert nb > 0, lib.hz_last_error()
         ws = torch.empty(nb, dtype=torch.uint8, device=device)
         R = _t(Rr.astype(np.float32), device)
         E = _t(Er.astype(np.float32), device)
