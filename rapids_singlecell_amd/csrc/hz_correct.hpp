@@ -394,9 +394,21 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(ApplyArgs<T> a) {
   const int b = a.ch_seg[c];
   const int p0 = a.ch_start[c], len = a.ch_len[c];
   const T* Wb = a.Wt + (long long)b * K * d;
-  for (int e = tid; e < K * Dp; e += kBlock) {
-    const int k = e / Dp, cc = e - k * Dp;
-    Ws[e] = (cc < d) ? Wb[(long long)k * d + cc] : T(0);
+  // W of the batch: eight loads in flight per thread (clamped addresses,
+  // padding columns stored as 0)
+  for (int e0 = tid; e0 < K * Dp; e0 += 8 * kBlock) {
+    T w[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = min(e0 + u * kBlock, K * Dp - 1);
+      const int k = e / Dp, cc = e - k * Dp;
+      w[u] = Wb[(long long)k * d + min(cc, d - 1)];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = e0 + u * kBlock;
+      if (e < K * Dp) Ws[e] = (e % Dp < d) ? w[u] : T(0);
+    }
   }
   for (int r0 = 0; r0 < len; r0 += kTileRows) {
     const int rows = min(kTileRows, len - r0);
