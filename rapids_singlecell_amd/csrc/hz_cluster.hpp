@@ -599,10 +599,19 @@ __global__ __launch_bounds__(kBlock) void yreduce_kernel(YRedArgs<T> a) {
       T ss = T(0);
       T yv[2] = {T(0), T(0)};
       for (int c = lane, u = 0; c < d; c += kWave, ++u) {
+        // the slices' loads all in flight, then added in slice order
+        auto slice_at = [&](int q) {
+          return __builtin_bit_cast(double, (unsigned long long)load_agent(reinterpret_cast<const long long*>(
+                                                &a.Yslice[((long long)q * a.K + k) * d + c])));
+        };
+        double vq[kYSlices];
+#pragma unroll
+        for (int q = 0; q < kYSlices; ++q) vq[q] = slice_at(q < ns ? q : 0);
         double v = 0.0;
-        for (int q = 0; q < ns; ++q)
-          v += __builtin_bit_cast(double, (unsigned long long)load_agent(reinterpret_cast<const long long*>(
-                                              &a.Yslice[((long long)q * a.K + k) * d + c])));
+#pragma unroll
+        for (int q = 0; q < kYSlices; ++q)
+          if (q < ns) v += vq[q];
+        for (int q = kYSlices; q < ns; ++q) v += slice_at(q);
         T t = T(v);
         if (u < 2) yv[u] = t;
         a.Y[(long long)k * d + c] = t;
